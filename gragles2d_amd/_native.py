@@ -1,0 +1,144 @@
+"""ctypes bindings of libggx (include/ggx.h).
+
+The library is the in-tree gragles2d_amd/libggx.so built by gragles2d_amd.build.  Loading it
+never touches the GPU; a context does.  There is no fallback: if the library is missing the
+import of a context fails loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libggx.so")
+
+GGX_OK = 0
+ERRORS = {
+    -1: "GGX_E_CONFIG", -2: "GGX_E_HIP", -3: "GGX_E_OOM", -4: "GGX_E_BOUNDS",
+    -5: "GGX_E_STATE", -6: "GGX_E_ARG", -7: "GGX_E_IO",
+}
+PREC_FP64, PREC_FP32 = 0, 1
+BATCH_FUSED, BATCH_PERGRAIN = 0, 1
+CONV_AUTO, CONV_STENCIL, CONV_FFT = 0, 1, 2
+
+c_int = ctypes.c_int32
+c_dbl = ctypes.c_double
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [(name, c_int) for name in (
+        "start_time", "number_of_particles", "number_of_points_per_grain", "number_of_timesteps",
+        "breakup_number", "analysis_timestep", "plot_interval", "discrete_sampling_rate",
+        "domain_border_size", "microstructure_gen_mode", "lattice_type", "use_mobility_model",
+        "identify_twins", "use_magnetic_field", "is_isotropic_network", "use_texture",
+        "execute_in_parallel", "maximum_number_of_threads", "grid_coarsement", "convolution_mode",
+        "research_mode", "research_project", "neighbour_tracking", "use_stored_elastic_energy",
+        "grain_scheduler", "decouple_grains", "user_def_number_of_points_per_grain")] + [
+        (name, c_dbl) for name in (
+            "hagb_mobility", "hagb_energy", "physical_domain_size", "triple_point_drag",
+            "disloc_en_per_m", "max_misorientation", "grid_coarsement_gradient",
+            "constant_sector_radius", "interpolating_sector_radius", "gaussian_kernel_time_slope",
+            "box_default_stored_elastic_energy")] + [
+        ("read_from_filename", ctypes.c_char * 512), ("additional_filename", ctypes.c_char * 512)]
+
+
+class Derived(ctypes.Structure):
+    _fields_ = [("real_domain_size", c_int), ("ngridpoints", c_int), ("grid_blowup", c_int),
+                ("h", c_dbl), ("dt", c_dbl), ("delta", c_dbl), ("time_slope", c_dbl), ("ds", c_dbl)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", c_int), ("batching", c_int), ("conv_engine", c_int), ("device", c_int),
+                ("lds_budget", c_int), ("use_graph", c_int), ("arena_headroom", c_dbl)]
+
+
+class GrainInfo(ctypes.Structure):
+    _fields_ = [(name, c_int) for name in (
+        "present", "exists", "motion_regular", "intersects_boundary", "xmin", "xmax", "ymin", "ymax",
+        "id_xmin", "id_xmax", "id_ymin", "id_ymax", "contour_len", "n_neighbours", "n_compare",
+        "error")] + [(name, c_dbl) for name in ("volume", "perimeter", "energy", "mean_da")]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(name, c_dbl) for name in (
+        "convolution_ms", "comparison_ms", "levelset_ms", "redistancing_ms", "rest_ms", "total_ms")] + [
+        ("steps", ctypes.c_int64), ("grid_point_updates", ctypes.c_int64),
+        ("algorithmic_bytes", ctypes.c_int64), ("conv_bytes", ctypes.c_int64),
+        ("live_grains", c_int), ("max_box", c_int)]
+
+
+# every symbol include/ggx.h declares
+EXPORTS = (
+    "ggx_settings_default", "ggx_settings_read_xml", "ggx_settings_write_xml", "ggx_settings_derive",
+    "ggx_network_generate", "ggx_network_read", "ggx_network_write", "ggx_network_count",
+    "ggx_network_grain", "ggx_network_add", "ggx_network_new", "ggx_network_free",
+    "ggx_options_default", "ggx_create", "ggx_destroy", "ggx_last_error", "ggx_load_network",
+    "ggx_step", "ggx_sync", "ggx_loop", "ggx_current_grains", "ggx_max_id", "ggx_get_info",
+    "ggx_get_info_all", "ggx_get_phi", "ggx_get_idlocal", "ggx_get_contour", "ggx_get_neighbours",
+    "ggx_get_compare_list", "ggx_get_stats",
+)
+
+_lib = None
+
+
+class GgxError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load libggx.so (cached).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GgxError(f"{path} is missing: run gragles2d_amd.build first (no fallback path exists)")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    sig = {
+        "ggx_settings_default": (c_int, [P(Settings)]),
+        "ggx_settings_read_xml": (c_int, [ctypes.c_char_p, P(Settings)]),
+        "ggx_settings_write_xml": (c_int, [ctypes.c_char_p, P(Settings), ctypes.c_char_p, c_int, c_int]),
+        "ggx_settings_derive": (c_int, [P(Settings), P(Derived)]),
+        "ggx_network_generate": (c_int, [c_int, ctypes.c_uint64, c_dbl, P(vp)]),
+        "ggx_network_read": (c_int, [ctypes.c_char_p, c_int, P(vp)]),
+        "ggx_network_write": (c_int, [ctypes.c_char_p, vp]),
+        "ggx_network_count": (c_int, [vp]),
+        "ggx_network_grain": (c_int, [vp, c_int, P(c_int), P(c_int), P(P(c_dbl)), P(P(c_dbl))]),
+        "ggx_network_add": (c_int, [vp, c_int, c_int, P(c_dbl), P(c_dbl)]),
+        "ggx_network_new": (c_int, [P(vp)]),
+        "ggx_network_free": (None, [vp]),
+        "ggx_options_default": (c_int, [P(Options)]),
+        "ggx_create": (c_int, [P(Settings), P(Options), P(vp)]),
+        "ggx_destroy": (None, [vp]),
+        "ggx_last_error": (ctypes.c_char_p, [vp]),
+        "ggx_load_network": (c_int, [vp, vp]),
+        "ggx_step": (c_int, [vp, c_int]),
+        "ggx_sync": (c_int, [vp]),
+        "ggx_loop": (c_int, [vp]),
+        "ggx_current_grains": (c_int, [vp]),
+        "ggx_max_id": (c_int, [vp]),
+        "ggx_get_info": (c_int, [vp, c_int, P(GrainInfo)]),
+        "ggx_get_info_all": (c_int, [vp, P(GrainInfo), c_int]),
+        "ggx_get_phi": (c_int, [vp, c_int, P(c_dbl), c_int]),
+        "ggx_get_idlocal": (c_int, [vp, c_int, P(ctypes.c_uint32), c_int]),
+        "ggx_get_contour": (c_int, [vp, c_int, P(c_dbl), c_int]),
+        "ggx_get_neighbours": (c_int, [vp, c_int, P(c_int), P(c_dbl), c_int]),
+        "ggx_get_compare_list": (c_int, [vp, c_int, P(c_int), c_int]),
+        "ggx_get_stats": (c_int, [vp, P(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what, ctx=None):
+    if rc is not None and rc < 0:
+        msg = ERRORS.get(rc, str(rc))
+        if ctx:
+            detail = load().ggx_last_error(ctx)
+            if detail:
+                msg += ": " + detail.decode()
+        raise GgxError(f"{what} failed: {msg}")
+    return rc

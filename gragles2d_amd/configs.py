@@ -1,0 +1,34 @@
+"""The five workloads of the timestep (seeded microstructures, parameters.xml keys).
+
+Seed of workload k = 1000 + k.  R = int(sqrt(N) * NumberOfPointsPerGrain) as in
+grainhdl::setSimulationParameter (src/grainhdl.cpp:77-78).
+"""
+from dataclasses import dataclass
+
+
+@dataclass(frozen=True)
+class Workload:
+    name: str
+    grains: int
+    points_per_grain: int
+    border: int          # DomainBorderSize (grid_blowup)
+    steps: int           # NumberOfTimesteps
+    lognormal_sigma: float
+    isotropic: bool
+    seed: int
+
+    def settings_kw(self):
+        return dict(number_of_particles=self.grains, number_of_points_per_grain=self.points_per_grain,
+                    number_of_timesteps=self.steps, analysis_timestep=10, plot_interval=1,
+                    domain_border_size=self.border, microstructure_gen_mode=0, convolution_mode=2,
+                    is_isotropic_network=1 if self.isotropic else 0, breakup_number=0,
+                    discrete_sampling_rate=31, grain_scheduler=0, lattice_type=0)
+
+
+WORKLOADS = {
+    "plain-100": Workload("plain-100", 100, 20, 7, 20, 0.0, True, 1000),
+    "iso-2k": Workload("iso-2k", 2000, 20, 20, 200, 0.0, True, 1001),
+    "aniso-10k": Workload("aniso-10k", 10000, 20, 7, 300, 0.0, False, 1002),
+    "batch-100k": Workload("batch-100k", 100000, 20, 7, 100, 0.35, True, 1003),
+    "diff-500": Workload("diff-500", 500, 20, 7, 128, 0.0, False, 1004),
+}

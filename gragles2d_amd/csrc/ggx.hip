@@ -1,0 +1,825 @@
+// ggx.hip -- libggx: device context, arena management and the batched timestep.
+//
+// Replaces the bodies of the grainhdl phase loops (src/grainhdl.cpp:755-869, 1243-1256,
+// 1381-1394) with batched HIP launches.  Per step:
+//   k_prep      deletion of grains that vanished in the previous step, allocator reset
+//   k_lists     comparison lists (computeSecondOrderNeighbours)
+//   k_conv_*    convolution of every live box (executeConvolution)
+//   k_compare   comparison + set-comparison + new bounding box
+//   k_contour   marching squares, motion check, new box allocation, neighbours, volume
+//   k_redist    redistancing into the next arena
+// Grains are sorted by box size and launched in size buckets so that each bucket gets the
+// LDS its largest box needs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ggx.h"
+#include "ggx_internal.h"
+#include "kernels.h"
+
+using namespace ggx;
+
+namespace {
+
+const int kBucketEdges[] = {24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 1 << 30};
+constexpr int kNumBuckets = sizeof(kBucketEdges) / sizeof(kBucketEdges[0]);
+constexpr int kMaxKernN = 2048;  // largest box side with a precomputed circulant
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct ggx_ctx {
+    ggx_settings s;
+    ggx_derived d;
+    ggx_options opt;
+    std::string err;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int lds_max = 65536;
+    int cu_count = 0;
+    bool loaded = false;
+    int max_id = 0;
+    int nslots = 0;
+    int loop = 0;
+    size_t esize = 8;  // sizeof field element
+
+    // arenas (field type erased)
+    void* P[2] = {nullptr, nullptr};
+    void* Tt = nullptr;
+    uint32_t* I[2] = {nullptr, nullptr};
+    int cur = 0;       // P[cur] holds phi of the current layout
+    int icur = 0;      // I[icur] holds the IDLocal written by the previous step
+    long long field_cap = 0;
+    double2* pts = nullptr;
+    long long contour_cap = 0;
+
+    // per-grain arrays
+    Box *box = nullptr, *box_next = nullptr;
+    long long *off = nullptr, *off_next = nullptr, *off_prev = nullptr;
+    int* nprev = nullptr;
+    int *flags = nullptr, *errs = nullptr;
+    int *sec_cnt = nullptr, *sec = nullptr, *cmp_cnt = nullptr, *cmp = nullptr;
+    int4* newbox = nullptr;
+    long long* coff = nullptr;
+    int *ccap = nullptr, *clen = nullptr;
+    double *volume = nullptr, *perimeter = nullptr, *energy = nullptr, *meanda = nullptr;
+    int *nb_cnt = nullptr, *nb_id = nullptr;
+    double* nb_len = nullptr;
+    unsigned long long* tops = nullptr;
+    int* live = nullptr;
+    int* nprev_tmp = nullptr;
+
+    // circulant kernels
+    long long* kern_off = nullptr;
+    void* kern = nullptr;
+    std::vector<char> kern_ready;
+
+    // host mirrors
+    std::vector<int> h_flags;
+    std::vector<Box> h_box;
+    std::vector<int> h_live;
+    std::vector<int> h_err;
+
+    // stats
+    hipEvent_t ev[8];
+    ggx_stats st;
+};
+
+namespace {
+
+int fail(ggx_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, GGX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <class X>
+int dalloc(ggx_ctx* ctx, X*& p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(X));
+    if (e != hipSuccess) return fail(ctx, GGX_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    p = (X*)q;
+    return GGX_OK;
+}
+
+template <class X>
+void dfree(X*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int contour_cap_of(int n) {
+    long long c = 16LL * n + 64;
+    long long c2 = 2LL * (n + 1) * (n + 1) + 2;
+    return (int)std::min(c, c2);
+}
+
+long long aligned(long long v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// K_n[m] = sum_k g(k) cos(2 pi k m / n), g(k) = exp(-c min(k,n-k)^2) / n with the Gaussian
+// exponent of box.cpp:682 split per dimension: c = sqrt(2/pi) 4 dt R^2 pi^2 / n^2.
+void circulant(int n, double dt, int R, std::vector<long double>& K) {
+    const long double PI = 3.14159265358979323846264338327950288L;
+    K.assign((size_t)n, 0.0L);
+    long double c = std::sqrt(2.0L / PI) * 4.0L * (long double)dt * (long double)R * (long double)R * PI * PI /
+                    ((long double)n * (long double)n);
+    std::vector<long double> g((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        int k2 = std::min(k, n - k);
+        g[k] = std::exp(-c * (long double)k2 * (long double)k2) / (long double)n;
+    }
+    for (int m = 0; m < n; ++m) {
+        long double s = 0.0L;
+        for (int k = 0; k < n; ++k) {
+            long long e = ((long long)k * m) % n;
+            s += g[k] * std::cos(2.0L * PI * (long double)e / (long double)n);
+        }
+        K[m] = s;
+    }
+}
+
+int ensure_kernels(ggx_ctx* ctx, int nmax) {
+    if (nmax >= kMaxKernN) return fail(ctx, GGX_E_BOUNDS, "box side exceeds the circulant table");
+    std::vector<long double> K;
+    for (int n = 1; n <= nmax; ++n) {
+        if (ctx->kern_ready[n]) continue;
+        circulant(n, ctx->d.dt, ctx->d.real_domain_size, K);
+        long long o = (long long)n * (n - 1) / 2;
+        if (ctx->esize == 8) {
+            std::vector<double> v(K.begin(), K.end());
+            HIPCHK(hipMemcpy((double*)ctx->kern + o, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> v(K.begin(), K.end());
+            HIPCHK(hipMemcpy((float*)ctx->kern + o, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+        }
+        ctx->kern_ready[n] = 1;
+    }
+    return GGX_OK;
+}
+
+GridConst grid_const(const ggx_ctx* ctx) {
+    GridConst G;
+    G.m = ctx->d.ngridpoints;
+    G.gb = ctx->d.grid_blowup;
+    G.R = ctx->d.real_domain_size;
+    G.nslots = ctx->nslots;
+    G.h = ctx->d.h;
+    G.delta = ctx->d.delta;
+    G.dt = ctx->d.dt;
+    return G;
+}
+
+int raise_lds(ggx_ctx* ctx) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, ctx->dev));
+    ctx->cu_count = prop.multiProcessorCount;
+    int lim = (int)prop.sharedMemPerBlock;
+    int smp = (int)prop.maxSharedMemoryPerMultiProcessor;
+    if (smp > lim) lim = smp;
+    if (ctx->opt.lds_budget > 0) lim = std::min(lim, ctx->opt.lds_budget);
+    // allow the dynamic-LDS kernels to use up to `lim`
+    const void* fns[] = {(const void*)k_conv_circulant<double>, (const void*)k_conv_circulant<float>,
+                         (const void*)k_contour<double>,        (const void*)k_contour<float>,
+                         (const void*)k_redist<double>,         (const void*)k_redist<float>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            lim = std::min(lim, (int)prop.sharedMemPerBlock);
+        }
+    }
+    ctx->lds_max = lim;
+    return GGX_OK;
+}
+
+struct Bucket {
+    int start, count, nmax;
+};
+
+std::vector<Bucket> make_buckets(const std::vector<int>& live, const std::vector<Box>& box) {
+    // live is sorted by n descending
+    std::vector<Bucket> out;
+    size_t i = 0;
+    while (i < live.size()) {
+        int n0 = box[live[i]].n;
+        int b = 0;
+        while (kBucketEdges[b] < n0) b++;
+        int lo = (b == 0) ? 0 : kBucketEdges[b - 1];
+        size_t j = i;
+        while (j < live.size() && box[live[j]].n > lo) j++;
+        out.push_back(Bucket{(int)i, (int)(j - i), n0});
+        i = j;
+    }
+    return out;
+}
+
+template <class T>
+int launch_step(ggx_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    GridConst G = grid_const(ctx);
+    std::vector<int>& live = ctx->h_live;
+    int nlive = (int)live.size();
+    const std::vector<Bucket> buckets = make_buckets(live, ctx->h_box);
+    bool pergrain = ctx->opt.batching == GGX_BATCH_PERGRAIN;
+    T* P = (T*)ctx->P[ctx->cur];
+    T* Pn = (T*)ctx->P[ctx->cur ^ 1];
+    T* Tt = (T*)ctx->Tt;
+    uint32_t* Iold = ctx->I[ctx->icur];
+    uint32_t* Inew = ctx->I[ctx->icur ^ 1];
+    int lds_max = ctx->lds_max;
+
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    hipLaunchKernelGGL(k_prep, dim3((ctx->nslots + 255) / 256), dim3(256), 0, st, ctx->nslots, ctx->flags, ctx->tops);
+    if (nlive > 0) {
+        if (pergrain) {
+            for (int i = 0; i < nlive; ++i)
+                hipLaunchKernelGGL(k_lists, dim3(1), dim3(64), 0, st, 1, ctx->live + i, ctx->box, ctx->flags,
+                                   ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
+        } else {
+            hipLaunchKernelGGL(k_lists, dim3((nlive + 63) / 64), dim3(64), 0, st, nlive, ctx->live, ctx->box,
+                               ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
+        }
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    // convolution
+    for (const Bucket& bk : buckets) {
+        int n = bk.nmax;
+        long long kk = ((2LL * n + 4 + 1) & ~1LL);
+        long long need = (kk + 2LL * n * n) * (long long)sizeof(T);
+        long long lds = std::min<long long>(need, lds_max);
+        if ((kk + 4LL * n) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
+        int grid = pergrain ? 1 : bk.count;
+        int reps = pergrain ? bk.count : 1;
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_conv_circulant<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r,
+                               ctx->box, ctx->off, (const T*)P, Tt, ctx->kern_off, (const T*)ctx->kern, (int)lds);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    // comparison
+    for (const Bucket& bk : buckets) {
+        int grid = pergrain ? 1 : bk.count;
+        int reps = pergrain ? bk.count : 1;
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_compare<T>, dim3(grid), dim3(256), 0, st, ctx->live + bk.start + r, ctx->box,
+                               ctx->off, ctx->cmp_cnt, ctx->cmp, (const T*)Tt, P, Iold, Inew, ctx->off_prev, ctx->nprev,
+                               ctx->newbox, ctx->flags, G);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    // level set
+    ContourOut O;
+    O.pts = ctx->pts; O.coff = ctx->coff; O.ccap = ctx->ccap; O.clen = ctx->clen;
+    O.volume = ctx->volume; O.perimeter = ctx->perimeter; O.energy = ctx->energy; O.meanda = ctx->meanda;
+    O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
+    O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
+    O.tops = ctx->tops; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
+    for (const Bucket& bk : buckets) {
+        int n = bk.nmax;
+        long long nwords = ((long long)n * n + 31) / 32;
+        int cc = contour_cap_of(n);
+        long long lds = ((nwords * 4 + (long long)cc * 4 + 15) & ~15LL) + (long long)cc * 8 + 16;
+        if (lds > lds_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the contour kernel");
+        int grid = pergrain ? 1 : bk.count;
+        int reps = pergrain ? bk.count : 1;
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_contour<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r,
+                               ctx->box, ctx->off, P, Inew, ctx->newbox, O, G, ctx->loop, ctx->s.analysis_timestep);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[4], st));
+    // redistancing
+    for (const Bucket& bk : buckets) {
+        int n = bk.nmax + 8;
+        long long lds = (long long)(n | 1) * n * (long long)sizeof(T);
+        if (lds > lds_max) lds = 0;
+        int grid = pergrain ? 1 : bk.count;
+        int reps = pergrain ? bk.count : 1;
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_redist<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r, ctx->box,
+                               ctx->off, (const T*)P, ctx->box_next, ctx->off_next, ctx->flags, Pn, G, (int)lds);
+    }
+    HIPCHK(hipEventRecord(ctx->ev[5], st));
+    HIPCHK(hipGetLastError());
+    return GGX_OK;
+}
+
+// Host-side copy of the per-grain state needed to schedule the next step.
+int pull_state(ggx_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->h_flags.data(), ctx->flags, ctx->nslots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctx->h_box.data(), ctx->box, ctx->nslots * sizeof(Box), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctx->h_err.data(), ctx->errs, ctx->nslots * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int g = 1; g < ctx->nslots; ++g)
+        if (ctx->h_err[g]) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "grain %d: device error %d at loop %d", g, ctx->h_err[g], ctx->loop);
+            return fail(ctx, GGX_E_BOUNDS, buf);
+        }
+    return GGX_OK;
+}
+
+void build_live(ggx_ctx* ctx) {
+    ctx->h_live.clear();
+    for (int g = 1; g < ctx->nslots; ++g)
+        if ((ctx->h_flags[g] & kPresent) && (ctx->h_flags[g] & kExists)) ctx->h_live.push_back(g);
+    const std::vector<Box>& b = ctx->h_box;
+    std::stable_sort(ctx->h_live.begin(), ctx->h_live.end(), [&](int a, int c) { return b[a].n > b[c].n; });
+}
+
+}  // namespace
+
+extern "C" int ggx_options_default(ggx_options* o) {
+    if (!o) return GGX_E_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = GGX_PREC_FP64;
+    o->batching = GGX_BATCH_FUSED;
+    o->conv_engine = GGX_CONV_AUTO;
+    o->device = 0;
+    o->lds_budget = 0;
+    o->use_graph = 0;
+    o->arena_headroom = 1.5;
+    return GGX_OK;
+}
+
+extern "C" int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx** out) {
+    if (!s || !out) return GGX_E_ARG;
+    ggx_ctx* ctx = new ggx_ctx();
+    ctx->s = *s;
+    if (o) ctx->opt = *o; else ggx_options_default(&ctx->opt);
+    int rc = ggx_settings_derive(s, &ctx->d);
+    if (rc != GGX_OK) { delete ctx; return rc; }
+    if (s->is_isotropic_network == 0) { delete ctx; return GGX_E_CONFIG; }  // anisotropic weights: not in this build
+    if (s->convolution_mode != 2) { delete ctx; return GGX_E_CONFIG; }     // Laplace/Richardson: FFT engine only
+    if (s->grid_coarsement || s->decouple_grains || s->use_magnetic_field || s->use_stored_elastic_energy) {
+        delete ctx;
+        return GGX_E_CONFIG;
+    }
+    ctx->esize = ctx->opt.precision == GGX_PREC_FP32 ? 4 : 8;
+    ctx->dev = ctx->opt.device;
+    hipError_t e = hipSetDevice(ctx->dev);
+    if (e != hipSuccess) { delete ctx; return GGX_E_HIP; }
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return GGX_E_HIP; }
+    for (auto& ev : ctx->ev)
+        if (hipEventCreate(&ev) != hipSuccess) { delete ctx; return GGX_E_HIP; }
+    if (raise_lds(ctx) != GGX_OK) { delete ctx; return GGX_E_HIP; }
+    std::memset(&ctx->st, 0, sizeof(ctx->st));
+    ctx->loop = s->start_time;
+    *out = ctx;
+    return GGX_OK;
+}
+
+extern "C" void ggx_destroy(ggx_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipDeviceSynchronize();
+    for (int k = 0; k < 2; ++k) { dfree(ctx->P[k]); dfree(ctx->I[k]); }
+    dfree(ctx->Tt); dfree(ctx->pts);
+    dfree(ctx->box); dfree(ctx->box_next); dfree(ctx->off); dfree(ctx->off_next); dfree(ctx->off_prev);
+    dfree(ctx->nprev); dfree(ctx->flags); dfree(ctx->errs); dfree(ctx->sec_cnt); dfree(ctx->sec);
+    dfree(ctx->cmp_cnt); dfree(ctx->cmp); dfree(ctx->newbox); dfree(ctx->coff); dfree(ctx->ccap); dfree(ctx->clen);
+    dfree(ctx->volume); dfree(ctx->perimeter); dfree(ctx->energy); dfree(ctx->meanda); dfree(ctx->nb_cnt);
+    dfree(ctx->nb_id); dfree(ctx->nb_len); dfree(ctx->tops); dfree(ctx->live); dfree(ctx->kern_off); dfree(ctx->kern);
+    for (auto& ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* ggx_last_error(const ggx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int ggx_load_network(ggx_ctx* ctx, const ggx_network* net) {
+    if (!ctx || !net) return GGX_E_ARG;
+    if (ctx->loaded) return fail(ctx, GGX_E_STATE, "network already loaded");
+    const ggx_derived& D = ctx->d;
+    int ng = (int)net->ids.size();
+    int max_id = 0;
+    for (int i = 0; i < ng; ++i) max_id = std::max(max_id, net->ids[i]);
+    max_id = std::max(max_id, ctx->s.number_of_particles);
+    ctx->max_id = max_id;
+    ctx->nslots = max_id + 1;
+    int NS = ctx->nslots;
+    int m = D.ngridpoints, gb = D.grid_blowup;
+    double h = D.h;
+    // LSbox file constructor (box.cpp:108-158) per grain
+    std::vector<Box> hbox((size_t)NS, Box{0, 0, 0, 0});
+    std::vector<int> hflags((size_t)NS, 0);
+    std::vector<double> hvol((size_t)NS, 0.0);
+    std::vector<int> hclen((size_t)NS, 0);
+    std::vector<int> voff((size_t)NS, 0), vcnt((size_t)NS, 0);
+    std::vector<double2> verts;
+    for (int i = 0; i < ng; ++i) {
+        int id = net->ids[i];
+        const std::vector<double>& xy = net->xy[i];
+        int nv = (int)(xy.size() / 2);
+        if (nv < 2) return fail(ctx, GGX_E_ARG, "grain with fewer than two vertices");
+        std::vector<double2> v((size_t)nv);
+        for (int k = 0; k < nv; ++k) {
+            double x = xy[2 * k], y = xy[2 * k + 1];
+            if (x < 0) x = 0;
+            if (y < 0) y = 0;
+            if (x > 1) x = 1;
+            if (y > 1) y = 1;
+            v[k] = make_double2(x, y);
+        }
+        int xmax = 0, xmin = m, ymax = 0, ymin = xmin;
+        for (int k = 0; k < nv; ++k) {
+            double y = v[k].y, x = v[k].x;
+            if (y / h < ymin) ymin = (int)(y / h);
+            if (y / h > ymax) ymax = (int)(y / h);
+            if (x / h < xmin) xmin = (int)(x / h);
+            if (x / h > xmax) xmax = (int)(x / h);
+        }
+        xmax += 2 * gb;
+        ymax += 2 * gb;
+        if (ymax > m) ymax = m;
+        if (xmax > m) xmax = m;
+        // resizeToSquare (dimensionalBuffer.h:123-160)
+        int grid_size = m - 1;
+        unsigned int height = (unsigned int)(ymax - ymin), width = (unsigned int)(xmax - xmin);
+        if (width != height) {
+            if (width > height) { int diff = (int)(width - height); ymin -= diff / 2; ymax += diff / 2 + diff % 2; }
+            else { int diff = (int)(height - width); xmin -= diff / 2; xmax += diff / 2 + diff % 2; }
+            if (xmin < 0) { int dl = -xmin; xmax += dl; xmin += dl; }
+            else if (xmax > grid_size) { int dl = grid_size - xmax; xmax += dl; xmin += dl; }
+            if (ymin < 0) { int dl = -ymin; ymax += dl; ymin += dl; }
+            else if (ymax > grid_size) { int dl = grid_size - ymax; ymax += dl; ymin += dl; }
+        }
+        hbox[id] = Box{xmin, ymin, xmax - xmin, ymax - ymin};
+        hflags[id] = kPresent | kExists | kRegular;
+        // m_volume = computeVolume() / (h*h) on the raw polygon (box.cpp:399-400)
+        double vs = 0;
+        for (int k = 0; k + 1 < nv; ++k) vs += (v[k].y + v[k + 1].y) * (v[k].x - v[k + 1].x);
+        double vol = std::fabs(vs) * h * h * 0.5;
+        hvol[id] = vol / (h * h);
+        hclen[id] = nv;
+        voff[id] = (int)verts.size();
+        vcnt[id] = nv;
+        verts.insert(verts.end(), v.begin(), v.end());
+    }
+    hflags[0] = kPresent | kExists;
+    // layout
+    std::vector<long long> hoff((size_t)NS, 0);
+    long long total = 0, ctotal = 0;
+    int nmax = 1;
+    for (int g = 1; g < NS; ++g) {
+        if (!(hflags[g] & kPresent)) continue;
+        if (hbox[g].n != hbox[g].pad) return fail(ctx, GGX_E_CONFIG, "non-square box after resizeToSquare");
+        hoff[g] = total;
+        total += aligned((long long)hbox[g].n * hbox[g].n);
+        ctotal += contour_cap_of(hbox[g].n);
+        nmax = std::max(nmax, hbox[g].n);
+    }
+    double hr = std::max(1.1, ctx->opt.arena_headroom);
+    ctx->field_cap = (long long)(hr * (double)total) + 64LL * 1024;
+    ctx->contour_cap = (long long)(hr * (double)ctotal) + 64LL * 1024;
+    // initial neighbour lists: closed overlap of the output boxes (find_neighbors,
+    // grainhdl.cpp:1258-1283; RTree Overlap), ascending ID
+    std::vector<int> hsec_cnt((size_t)NS, 0), hsec((size_t)NS * kCmpCap, 0);
+    {
+        int bsz = std::max(16, nmax);
+        int nb = (m + bsz - 1) / bsz + 1;
+        std::vector<std::vector<int>> bins((size_t)nb * nb);
+        for (int g = 1; g < NS; ++g) {
+            if (!(hflags[g] & kPresent)) continue;
+            const Box& b = hbox[g];
+            for (int by = b.y0 / bsz; by <= std::min(nb - 1, (b.y0 + b.n) / bsz); ++by)
+                for (int bx = b.x0 / bsz; bx <= std::min(nb - 1, (b.x0 + b.n) / bsz); ++bx)
+                    bins[(size_t)by * nb + bx].push_back(g);
+        }
+        std::vector<int> cand;
+        for (int g = 1; g < NS; ++g) {
+            if (!(hflags[g] & kPresent)) continue;
+            const Box& a = hbox[g];
+            cand.clear();
+            for (int by = a.y0 / bsz; by <= std::min(nb - 1, (a.y0 + a.n) / bsz); ++by)
+                for (int bx = a.x0 / bsz; bx <= std::min(nb - 1, (a.x0 + a.n) / bsz); ++bx)
+                    for (int o : bins[(size_t)by * nb + bx]) cand.push_back(o);
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            int cnt = 0;
+            for (int o : cand) {
+                if (o == g) continue;
+                const Box& b = hbox[o];
+                if (a.x0 > b.x0 + b.n || b.x0 > a.x0 + a.n || a.y0 > b.y0 + b.n || b.y0 > a.y0 + a.n) continue;
+                if (cnt >= kCmpCap) return fail(ctx, GGX_E_BOUNDS, "initial neighbour list over capacity");
+                hsec[(size_t)g * kCmpCap + cnt++] = o;
+            }
+            hsec_cnt[g] = cnt;
+        }
+    }
+    // the initial direct-neighbour list holds the same grains with zero length
+    // (LSbox::computeDirectNeighbours -> addDirectNeighbourManual, box.cpp:2220-2237)
+    std::vector<int> hnb_cnt((size_t)NS, 0), hnb_id((size_t)NS * kNbCap, 0);
+    for (int g = 1; g < NS; ++g) {
+        int c = std::min(hsec_cnt[g], kNbCap);
+        hnb_cnt[g] = c;
+        for (int k = 0; k < c; ++k) hnb_id[(size_t)g * kNbCap + k] = hsec[(size_t)g * kCmpCap + k];
+    }
+
+    // device allocations
+    size_t es = ctx->esize;
+    void* p;
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipMalloc(&p, (size_t)ctx->field_cap * es)); ctx->P[k] = p;
+        HIPCHK(hipMalloc(&p, (size_t)ctx->field_cap * 4)); ctx->I[k] = (uint32_t*)p;
+        HIPCHK(hipMemset(ctx->I[k], 0, (size_t)ctx->field_cap * 4));
+    }
+    HIPCHK(hipMalloc(&p, (size_t)ctx->field_cap * es)); ctx->Tt = p;
+    int rc;
+    if ((rc = dalloc(ctx, ctx->pts, (size_t)ctx->contour_cap))) return rc;
+    if ((rc = dalloc(ctx, ctx->box, NS)) || (rc = dalloc(ctx, ctx->box_next, NS)) || (rc = dalloc(ctx, ctx->off, NS)) ||
+        (rc = dalloc(ctx, ctx->off_next, NS)) || (rc = dalloc(ctx, ctx->off_prev, NS)) || (rc = dalloc(ctx, ctx->nprev, NS)) ||
+        (rc = dalloc(ctx, ctx->flags, NS)) || (rc = dalloc(ctx, ctx->errs, NS)) || (rc = dalloc(ctx, ctx->sec_cnt, NS)) ||
+        (rc = dalloc(ctx, ctx->sec, (size_t)NS * kCmpCap)) || (rc = dalloc(ctx, ctx->cmp_cnt, NS)) ||
+        (rc = dalloc(ctx, ctx->cmp, (size_t)NS * kCmpCap)) || (rc = dalloc(ctx, ctx->newbox, NS)) ||
+        (rc = dalloc(ctx, ctx->coff, NS)) || (rc = dalloc(ctx, ctx->ccap, NS)) || (rc = dalloc(ctx, ctx->clen, NS)) ||
+        (rc = dalloc(ctx, ctx->volume, NS)) || (rc = dalloc(ctx, ctx->perimeter, NS)) || (rc = dalloc(ctx, ctx->energy, NS)) ||
+        (rc = dalloc(ctx, ctx->meanda, NS)) || (rc = dalloc(ctx, ctx->nb_cnt, NS)) ||
+        (rc = dalloc(ctx, ctx->nb_id, (size_t)NS * kNbCap)) || (rc = dalloc(ctx, ctx->nb_len, (size_t)NS * kNbCap)) ||
+        (rc = dalloc(ctx, ctx->tops, 4)) || (rc = dalloc(ctx, ctx->live, NS)) ||
+        (rc = dalloc(ctx, ctx->kern_off, kMaxKernN)))
+        return rc;
+    HIPCHK(hipMalloc(&p, (size_t)kMaxKernN * (kMaxKernN - 1) / 2 * es + 64)); ctx->kern = p;
+    ctx->kern_ready.assign(kMaxKernN, 0);
+    {
+        std::vector<long long> ko(kMaxKernN);
+        for (int n = 0; n < kMaxKernN; ++n) ko[n] = (long long)n * (n - 1) / 2;
+        HIPCHK(hipMemcpy(ctx->kern_off, ko.data(), ko.size() * 8, hipMemcpyHostToDevice));
+    }
+    if ((rc = ensure_kernels(ctx, nmax + 16))) return rc;
+
+    std::vector<int> hnprev((size_t)NS);
+    for (int g = 0; g < NS; ++g) hnprev[g] = hbox[g].n;
+    std::vector<double> zeros((size_t)NS * kNbCap, 0.0);
+    std::vector<int> izeros((size_t)NS, 0);
+    HIPCHK(hipMemcpy(ctx->box, hbox.data(), NS * sizeof(Box), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->box_next, hbox.data(), NS * sizeof(Box), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->off, hoff.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->off_prev, hoff.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->off_next, hoff.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->nprev, hnprev.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->flags, hflags.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->errs, izeros.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sec_cnt, hsec_cnt.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sec, hsec.data(), hsec.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->cmp_cnt, izeros.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->clen, hclen.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->ccap, izeros.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->volume, hvol.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->perimeter, zeros.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->energy, zeros.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->meanda, zeros.data(), NS * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->nb_cnt, hnb_cnt.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->nb_id, hnb_id.data(), hnb_id.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->nb_len, zeros.data(), zeros.size() * 8, hipMemcpyHostToDevice));
+
+    // initial signed distance (distanceInitialisation, grainhdl.cpp:728-753)
+    ctx->h_flags = hflags;
+    ctx->h_box = hbox;
+    ctx->h_err.assign((size_t)NS, 0);
+    build_live(ctx);
+    int nlive = (int)ctx->h_live.size();
+    int *d_voff = nullptr, *d_vcnt = nullptr;
+    double2* d_verts = nullptr;
+    if ((rc = dalloc(ctx, d_voff, NS)) || (rc = dalloc(ctx, d_vcnt, NS)) || (rc = dalloc(ctx, d_verts, verts.size())))
+        return rc;
+    HIPCHK(hipMemcpy(d_voff, voff.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_vcnt, vcnt.data(), NS * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_verts, verts.data(), verts.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->live, ctx->h_live.data(), nlive * 4, hipMemcpyHostToDevice));
+    GridConst G = grid_const(ctx);
+    if (nlive > 0) {
+        if (es == 8)
+            hipLaunchKernelGGL(k_init_sdf<double>, dim3(nlive), dim3(256), 0, ctx->stream, ctx->live, ctx->box, ctx->off,
+                               d_voff, d_vcnt, d_verts, (double*)ctx->P[ctx->cur], G);
+        else
+            hipLaunchKernelGGL(k_init_sdf<float>, dim3(nlive), dim3(256), 0, ctx->stream, ctx->live, ctx->box, ctx->off,
+                               d_voff, d_vcnt, d_verts, (float*)ctx->P[ctx->cur], G);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    dfree(d_voff); dfree(d_vcnt); dfree(d_verts);
+    ctx->loaded = true;
+    return GGX_OK;
+}
+
+extern "C" int ggx_step(ggx_ctx* ctx, int32_t nsteps) {
+    if (!ctx) return GGX_E_ARG;
+    if (!ctx->loaded) return fail(ctx, GGX_E_STATE, "ggx_step before ggx_load_network");
+    for (int s = 0; s < nsteps; ++s) {
+        build_live(ctx);
+        int nlive = (int)ctx->h_live.size();
+        int nmax = 1;
+        long long upd = 0;
+        for (int g : ctx->h_live) { nmax = std::max(nmax, ctx->h_box[g].n); upd += (long long)ctx->h_box[g].n * ctx->h_box[g].n; }
+        int rc = ensure_kernels(ctx, nmax);
+        if (rc) return rc;
+        if (nlive > 0) HIPCHK(hipMemcpyAsync(ctx->live, ctx->h_live.data(), nlive * 4, hipMemcpyHostToDevice, ctx->stream));
+        rc = (ctx->esize == 8) ? launch_step<double>(ctx) : launch_step<float>(ctx);
+        if (rc) return rc;
+        // layout switch: next arena becomes current; this step's IDLocal becomes the stale source
+        HIPCHK(hipMemcpyAsync(ctx->off_prev, ctx->off, ctx->nslots * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        {
+            // n_prev = n of this step's box (gathered through a small copy of box.n)
+        }
+        std::swap(ctx->box, ctx->box_next);
+        std::swap(ctx->off, ctx->off_next);
+        std::swap(ctx->sec, ctx->cmp);
+        std::swap(ctx->sec_cnt, ctx->cmp_cnt);
+        ctx->cur ^= 1;
+        ctx->icur ^= 1;
+        rc = pull_state(ctx);
+        // n_prev for the next step = side of the box this step used
+        {
+            // the previous box array is now box_next; copy its n field on the host
+            std::vector<Box> prevbox((size_t)ctx->nslots);
+            HIPCHK(hipMemcpy(prevbox.data(), ctx->box_next, ctx->nslots * sizeof(Box), hipMemcpyDeviceToHost));
+            std::vector<int> np((size_t)ctx->nslots);
+            for (int g = 0; g < ctx->nslots; ++g) np[g] = prevbox[g].n;
+            HIPCHK(hipMemcpy(ctx->nprev, np.data(), ctx->nslots * 4, hipMemcpyHostToDevice));
+        }
+        float ms[5];
+        for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]));
+        ctx->st.comparison_ms += ms[0] + ms[2];
+        ctx->st.convolution_ms += ms[1];
+        ctx->st.levelset_ms += ms[3];
+        ctx->st.redistancing_ms += ms[4];
+        ctx->st.total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
+        ctx->st.steps += 1;
+        ctx->st.grid_point_updates = upd;
+        ctx->st.live_grains = nlive;
+        ctx->st.max_box = nmax;
+        ctx->loop += 1;
+        if (rc) return rc;
+    }
+    return GGX_OK;
+}
+
+extern "C" int ggx_sync(ggx_ctx* ctx) {
+    if (!ctx) return GGX_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GGX_OK;
+}
+
+extern "C" int ggx_loop(const ggx_ctx* ctx) { return ctx ? ctx->loop : GGX_E_ARG; }
+extern "C" int ggx_max_id(const ggx_ctx* ctx) { return ctx ? ctx->max_id : GGX_E_ARG; }
+
+extern "C" int ggx_current_grains(ggx_ctx* ctx) {
+    if (!ctx || !ctx->loaded) return GGX_E_STATE;
+    int c = 0;
+    for (int g = 1; g < ctx->nslots; ++g) c += (ctx->h_flags[g] & kPresent) ? 1 : 0;
+    return c;
+}
+
+namespace {
+template <class X>
+int fetch(ggx_ctx* ctx, const X* dsrc, X* h, size_t count) {
+    HIPCHK(hipMemcpy(h, dsrc, count * sizeof(X), hipMemcpyDeviceToHost));
+    return GGX_OK;
+}
+}  // namespace
+
+extern "C" int ggx_get_info(ggx_ctx* ctx, int32_t id, ggx_grain_info* o) {
+    if (!ctx || !o || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    std::memset(o, 0, sizeof(*o));
+    int f, e, cl, nbc, cc;
+    Box b;
+    double v, p, en, md;
+    int rc;
+    if ((rc = fetch(ctx, ctx->flags + id, &f, 1)) || (rc = fetch(ctx, ctx->errs + id, &e, 1)) ||
+        (rc = fetch(ctx, ctx->box + id, &b, 1)) || (rc = fetch(ctx, ctx->clen + id, &cl, 1)) ||
+        (rc = fetch(ctx, ctx->nb_cnt + id, &nbc, 1)) || (rc = fetch(ctx, ctx->sec_cnt + id, &cc, 1)) ||
+        (rc = fetch(ctx, ctx->volume + id, &v, 1)) || (rc = fetch(ctx, ctx->perimeter + id, &p, 1)) ||
+        (rc = fetch(ctx, ctx->energy + id, &en, 1)) || (rc = fetch(ctx, ctx->meanda + id, &md, 1)))
+        return rc;
+    Box bi;
+    if ((rc = fetch(ctx, ctx->box_next + id, &bi, 1))) return rc;  // the box the last step compared on
+    o->present = (f & kPresent) != 0;
+    o->exists = (f & kExists) != 0;
+    o->motion_regular = (f & kRegular) != 0;
+    o->intersects_boundary = (f & kBoundary) != 0;
+    o->xmin = b.x0; o->xmax = b.x0 + b.n; o->ymin = b.y0; o->ymax = b.y0 + b.n;
+    o->id_xmin = bi.x0; o->id_xmax = bi.x0 + bi.n; o->id_ymin = bi.y0; o->id_ymax = bi.y0 + bi.n;
+    o->contour_len = cl;
+    o->n_neighbours = nbc;
+    o->n_compare = cc;
+    o->error = e;
+    o->volume = v; o->perimeter = p; o->energy = en; o->mean_da = md;
+    return GGX_OK;
+}
+
+extern "C" int ggx_get_info_all(ggx_ctx* ctx, ggx_grain_info* out, int32_t count) {
+    if (!ctx || !out || !ctx->loaded) return GGX_E_ARG;
+    int NS = std::min(count, ctx->nslots);
+    std::vector<int> f(NS), e(NS), cl(NS), nbc(NS), cc(NS);
+    std::vector<Box> b(NS), bi(NS);
+    std::vector<double> v(NS), p(NS), en(NS), md(NS);
+    int rc;
+    if ((rc = fetch(ctx, ctx->flags, f.data(), NS)) || (rc = fetch(ctx, ctx->errs, e.data(), NS)) ||
+        (rc = fetch(ctx, ctx->box, b.data(), NS)) || (rc = fetch(ctx, ctx->box_next, bi.data(), NS)) ||
+        (rc = fetch(ctx, ctx->clen, cl.data(), NS)) || (rc = fetch(ctx, ctx->nb_cnt, nbc.data(), NS)) ||
+        (rc = fetch(ctx, ctx->sec_cnt, cc.data(), NS)) || (rc = fetch(ctx, ctx->volume, v.data(), NS)) ||
+        (rc = fetch(ctx, ctx->perimeter, p.data(), NS)) || (rc = fetch(ctx, ctx->energy, en.data(), NS)) ||
+        (rc = fetch(ctx, ctx->meanda, md.data(), NS)))
+        return rc;
+    for (int id = 0; id < NS; ++id) {
+        ggx_grain_info& o = out[id];
+        std::memset(&o, 0, sizeof(o));
+        o.present = (f[id] & kPresent) != 0;
+        o.exists = (f[id] & kExists) != 0;
+        o.motion_regular = (f[id] & kRegular) != 0;
+        o.intersects_boundary = (f[id] & kBoundary) != 0;
+        o.xmin = b[id].x0; o.xmax = b[id].x0 + b[id].n; o.ymin = b[id].y0; o.ymax = b[id].y0 + b[id].n;
+        o.id_xmin = bi[id].x0; o.id_xmax = bi[id].x0 + bi[id].n; o.id_ymin = bi[id].y0; o.id_ymax = bi[id].y0 + bi[id].n;
+        o.contour_len = cl[id];
+        o.n_neighbours = nbc[id];
+        o.n_compare = cc[id];
+        o.error = e[id];
+        o.volume = v[id]; o.perimeter = p[id]; o.energy = en[id]; o.mean_da = md[id];
+    }
+    return NS;
+}
+
+extern "C" int ggx_get_phi(ggx_ctx* ctx, int32_t id, double* out, int32_t cap) {
+    if (!ctx || !out || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    Box b;
+    long long o;
+    int rc;
+    if ((rc = fetch(ctx, ctx->box + id, &b, 1)) || (rc = fetch(ctx, ctx->off + id, &o, 1))) return rc;
+    int cnt = b.n * b.n;
+    if (cnt > cap) return GGX_E_ARG;
+    if (ctx->esize == 8) {
+        if ((rc = fetch(ctx, (const double*)ctx->P[ctx->cur] + o, out, cnt))) return rc;
+    } else {
+        std::vector<float> tmp(cnt);
+        if ((rc = fetch(ctx, (const float*)ctx->P[ctx->cur] + o, tmp.data(), cnt))) return rc;
+        for (int k = 0; k < cnt; ++k) out[k] = tmp[k];
+    }
+    return cnt;
+}
+
+extern "C" int ggx_get_idlocal(ggx_ctx* ctx, int32_t id, uint32_t* out, int32_t cap) {
+    if (!ctx || !out || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    // IDLocal of the last step lives in I[icur] with the layout of that step (off_prev, box_next)
+    Box b;
+    long long o;
+    int rc;
+    if ((rc = fetch(ctx, ctx->box_next + id, &b, 1)) || (rc = fetch(ctx, ctx->off_prev + id, &o, 1))) return rc;
+    int cnt = b.n * b.n;
+    if (cnt > cap) return GGX_E_ARG;
+    if ((rc = fetch(ctx, ctx->I[ctx->icur] + o, out, cnt))) return rc;
+    return cnt;
+}
+
+extern "C" int ggx_get_contour(ggx_ctx* ctx, int32_t id, double* xy, int32_t cap_points) {
+    if (!ctx || !xy || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    int len;
+    long long co;
+    int rc;
+    if ((rc = fetch(ctx, ctx->clen + id, &len, 1)) || (rc = fetch(ctx, ctx->coff + id, &co, 1))) return rc;
+    if (len > cap_points) return GGX_E_ARG;
+    if ((rc = fetch(ctx, ctx->pts + co, (double2*)xy, len))) return rc;
+    return len;
+}
+
+extern "C" int ggx_get_neighbours(ggx_ctx* ctx, int32_t id, int32_t* ids, double* lengths, int32_t cap) {
+    if (!ctx || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    int c;
+    int rc;
+    if ((rc = fetch(ctx, ctx->nb_cnt + id, &c, 1))) return rc;
+    if (c > cap) return GGX_E_ARG;
+    if (ids && (rc = fetch(ctx, ctx->nb_id + (size_t)id * kNbCap, ids, c))) return rc;
+    if (lengths && (rc = fetch(ctx, ctx->nb_len + (size_t)id * kNbCap, lengths, c))) return rc;
+    return c;
+}
+
+extern "C" int ggx_get_compare_list(ggx_ctx* ctx, int32_t id, int32_t* ids, int32_t cap) {
+    if (!ctx || !ctx->loaded || id < 1 || id >= ctx->nslots) return GGX_E_ARG;
+    int c;
+    int rc;
+    if ((rc = fetch(ctx, ctx->sec_cnt + id, &c, 1))) return rc;
+    if (c > cap) return GGX_E_ARG;
+    if (ids && (rc = fetch(ctx, ctx->sec + (size_t)id * kCmpCap, ids, c))) return rc;
+    return c;
+}
+
+extern "C" int ggx_get_stats(ggx_ctx* ctx, ggx_stats* out) {
+    if (!ctx || !out) return GGX_E_ARG;
+    *out = ctx->st;
+    return GGX_OK;
+}

@@ -1,0 +1,43 @@
+// ggx_internal.h -- host-side types shared by the libggx translation units.
+#ifndef GGX_INTERNAL_H
+#define GGX_INTERNAL_H
+#include <array>
+#include <cstdint>
+#include <vector>
+
+// Mode-0 microstructure in host memory: one closed polygon (unit coordinates) and one
+// orientation quaternion per grain (grainhdl::readMicrostructure, src/grainhdl.cpp:605-661).
+struct ggx_network {
+    std::vector<int32_t> ids;
+    std::vector<std::vector<double>> xy;
+    std::vector<std::array<double, 4>> q;
+};
+
+namespace ggx {
+
+// Grain box, square by construction: [x0, x0+n) x [y0, y0+n) in global grid coordinates.
+struct Box {
+    int32_t x0, y0, n, pad;
+};
+
+// Limits of the fixed-capacity per-grain lists on the device.
+constexpr int kCmpCap = 128;     // comparison list (m_comparisonList / m_secondOrderNeighbours)
+constexpr int kNbCap = 64;       // direct neighbours (m_directNeighbourhood)
+constexpr int kAlign = 64;       // element alignment of every box in the arenas
+
+// Per-grain flags (bit set).
+constexpr int kPresent = 1;      // grains[id] != NULL
+constexpr int kExists = 2;       // LSbox::m_exists
+constexpr int kRegular = 4;      // LSbox::m_isMotionRegular
+constexpr int kBoundary = 8;     // LSbox::m_intersectsBoundaryGrain
+
+// Device error codes (per grain).
+constexpr int kErrContourCap = 1;   // contour longer than its arena slot
+constexpr int kErrTrace = 2;        // marching squares did not close within its iteration cap
+constexpr int kErrClearBounds = 3;  // clearContourGrainArea left the buffer (uncaught in ref)
+constexpr int kErrArena = 4;        // field arena exhausted
+constexpr int kErrCmpCap = 5;       // comparison list over capacity
+constexpr int kErrNbCap = 6;        // direct-neighbour list over capacity
+
+}  // namespace ggx
+#endif

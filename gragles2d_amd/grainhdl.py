@@ -1,0 +1,250 @@
+"""Host-side mirror of the reference driver (grainhdl / LSbox accessors) over libggx.
+
+The names follow src/grainhdl.h and src/box.h: ``GrainHandler.set_simulation_parameter``,
+``run_sim``, ``get_biggest_grain_vol``; grains expose ``get_volume``, ``get_perimeter``,
+``get_energy``, ``get_direct_neighbour_ids``, ``get_gb_lengths``, ``get_min_x`` ... .
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+
+class Network:
+    """Mode-0 microstructure (grainhdl::readMicrostructure, src/grainhdl.cpp:605-661)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._lib = N.load()
+
+    @classmethod
+    def generate(cls, n, seed, lognormal_sigma=0.0):
+        L = N.load()
+        h = ctypes.c_void_p()
+        N.check(L.ggx_network_generate(n, seed, lognormal_sigma, ctypes.byref(h)), "ggx_network_generate")
+        return cls(h)
+
+    @classmethod
+    def read(cls, path, ngrains):
+        L = N.load()
+        h = ctypes.c_void_p()
+        N.check(L.ggx_network_read(path.encode(), ngrains, ctypes.byref(h)), "ggx_network_read")
+        return cls(h)
+
+    @classmethod
+    def from_polygons(cls, polys, quats=None):
+        L = N.load()
+        h = ctypes.c_void_p()
+        N.check(L.ggx_network_new(ctypes.byref(h)), "ggx_network_new")
+        net = cls(h)
+        for i, p in enumerate(polys):
+            xy = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+            q = np.ascontiguousarray(quats[i] if quats is not None else [1.0, 0, 0, 0], dtype=np.float64)
+            N.check(L.ggx_network_add(h, i + 1, len(xy) // 2, xy.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "ggx_network_add")
+        return net
+
+    def write(self, path):
+        N.check(self._lib.ggx_network_write(path.encode(), self._h), "ggx_network_write")
+
+    def __len__(self):
+        return self._lib.ggx_network_count(self._h)
+
+    def grain(self, idx):
+        gid, nv = N.c_int(), N.c_int()
+        xy = ctypes.POINTER(ctypes.c_double)()
+        q = ctypes.POINTER(ctypes.c_double)()
+        N.check(self._lib.ggx_network_grain(self._h, idx, ctypes.byref(gid), ctypes.byref(nv), ctypes.byref(xy),
+                                            ctypes.byref(q)), "ggx_network_grain")
+        pts = np.ctypeslib.as_array(xy, shape=(nv.value * 2,)).reshape(-1, 2).copy()
+        quat = np.ctypeslib.as_array(q, shape=(4,)).copy()
+        return gid.value, pts, quat
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.ggx_network_free(self._h)
+            self._h = None
+
+
+def default_settings(**kw):
+    """Settings with the reference defaults (src/Settings.cpp:29-80) plus keyword overrides."""
+    L = N.load()
+    s = N.Settings()
+    N.check(L.ggx_settings_default(ctypes.byref(s)), "ggx_settings_default")
+    for k, v in kw.items():
+        if isinstance(v, str):
+            setattr(s, k, v.encode())
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def read_settings(path):
+    L = N.load()
+    s = N.Settings()
+    N.check(L.ggx_settings_read_xml(path.encode(), ctypes.byref(s)), "ggx_settings_read_xml")
+    return s
+
+
+def derive(settings):
+    L = N.load()
+    d = N.Derived()
+    N.check(L.ggx_settings_derive(ctypes.byref(settings), ctypes.byref(d)), "ggx_settings_derive")
+    return d
+
+
+class GrainView:
+    """Read-only LSbox view (src/box.h accessors)."""
+
+    def __init__(self, hdl, gid, info):
+        self._hdl, self.id, self.info = hdl, gid, info
+
+    def grain_exists(self):
+        return bool(self.info.exists)
+
+    def get_volume(self):
+        return self.info.volume
+
+    def get_perimeter(self):
+        return self.info.perimeter
+
+    def get_energy(self):
+        return self.info.energy
+
+    def get_min_x(self):
+        return self.info.xmin
+
+    def get_max_x(self):
+        return self.info.xmax
+
+    def get_min_y(self):
+        return self.info.ymin
+
+    def get_max_y(self):
+        return self.info.ymax
+
+    def is_motion_regular(self):
+        return bool(self.info.motion_regular)
+
+    def intersects_boundary_grain(self):
+        return bool(self.info.intersects_boundary)
+
+    def get_direct_neighbour_ids(self):
+        return self._hdl.neighbours(self.id)[0]
+
+    def get_gb_lengths(self):
+        return self._hdl.neighbours(self.id)[1]
+
+    def phi(self):
+        return self._hdl.phi(self.id)
+
+
+class GrainHandler:
+    """grainhdl over a libggx context: set_simulation_parameter + run_sim."""
+
+    def __init__(self, settings, precision=N.PREC_FP64, batching=N.BATCH_FUSED, device=0, **opts):
+        self.L = N.load()
+        self.settings = settings
+        self.derived = derive(settings)
+        o = N.Options()
+        N.check(self.L.ggx_options_default(ctypes.byref(o)), "ggx_options_default")
+        o.precision, o.batching, o.device = precision, batching, device
+        for k, v in opts.items():
+            setattr(o, k, v)
+        self.options = o
+        self._ctx = ctypes.c_void_p()
+        N.check(self.L.ggx_create(ctypes.byref(settings), ctypes.byref(o), ctypes.byref(self._ctx)), "ggx_create")
+        self.loop = settings.start_time
+
+    def _chk(self, rc, what):
+        return N.check(rc, what, self._ctx)
+
+    def set_simulation_parameter(self, network):
+        """grainhdl::setSimulationParameter's microstructure part: build boxes, initial distance
+        functions and neighbour lists on the device."""
+        self._chk(self.L.ggx_load_network(self._ctx, network._h), "ggx_load_network")
+        self.max_id = self.L.ggx_max_id(self._ctx)
+
+    def step(self, n=1):
+        self._chk(self.L.ggx_step(self._ctx, n), "ggx_step")
+        self.loop = self.L.ggx_loop(self._ctx)
+
+    def sync(self):
+        self._chk(self.L.ggx_sync(self._ctx), "ggx_sync")
+
+    def run_sim(self, nsteps=None):
+        """The run_sim loop (src/grainhdl.cpp:1049-1141): NumberOfTimesteps+1 passes, stopping
+        early when fewer than BreakupNumber grains remain."""
+        total = (self.settings.number_of_timesteps + 1) if nsteps is None else nsteps
+        for _ in range(total):
+            self.step(1)
+            if self.current_grains() < self.settings.breakup_number:
+                break
+
+    def current_grains(self):
+        return self._chk(self.L.ggx_current_grains(self._ctx), "ggx_current_grains")
+
+    def info(self, gid):
+        gi = N.GrainInfo()
+        self._chk(self.L.ggx_get_info(self._ctx, gid, ctypes.byref(gi)), "ggx_get_info")
+        return gi
+
+    def info_all(self):
+        n = self.max_id + 1
+        arr = (N.GrainInfo * n)()
+        self._chk(self.L.ggx_get_info_all(self._ctx, arr, n), "ggx_get_info_all")
+        return arr
+
+    def grain(self, gid):
+        return GrainView(self, gid, self.info(gid))
+
+    def phi(self, gid):
+        gi = self.info(gid)
+        n = gi.xmax - gi.xmin
+        out = np.zeros(max(n * n, 1), dtype=np.float64)
+        cnt = self._chk(self.L.ggx_get_phi(self._ctx, gid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size),
+                        "ggx_get_phi")
+        return out[:cnt].reshape(n, n)
+
+    def idlocal(self, gid):
+        gi = self.info(gid)
+        n = gi.id_xmax - gi.id_xmin
+        out = np.zeros(max(n * n, 1), dtype=np.uint32)
+        cnt = self._chk(self.L.ggx_get_idlocal(self._ctx, gid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               out.size), "ggx_get_idlocal")
+        return out[:cnt]
+
+    def contour(self, gid):
+        gi = self.info(gid)
+        out = np.zeros(max(2 * gi.contour_len, 2), dtype=np.float64)
+        cnt = self._chk(self.L.ggx_get_contour(self._ctx, gid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               gi.contour_len), "ggx_get_contour")
+        return out[:2 * cnt].reshape(-1, 2)
+
+    def neighbours(self, gid):
+        ids = np.zeros(256, dtype=np.int32)
+        lens = np.zeros(256, dtype=np.float64)
+        cnt = self._chk(self.L.ggx_get_neighbours(self._ctx, gid, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                  lens.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 256),
+                        "ggx_get_neighbours")
+        return ids[:cnt].copy(), lens[:cnt].copy()
+
+    def compare_list(self, gid):
+        ids = np.zeros(256, dtype=np.int32)
+        cnt = self._chk(self.L.ggx_get_compare_list(self._ctx, gid, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                    256), "ggx_get_compare_list")
+        return ids[:cnt].copy()
+
+    def stats(self):
+        st = N.Stats()
+        self._chk(self.L.ggx_get_stats(self._ctx, ctypes.byref(st)), "ggx_get_stats")
+        return st
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.L.ggx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()

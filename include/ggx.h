@@ -1,0 +1,217 @@
+/*
+ * ggx.h -- C-ABI of libggx, the MI355X-native per-grain level-set timestep of GraGLeS2D.
+ *
+ * The reference has no plugin API: its timestep is the body of grainhdl::run_sim
+ * (src/grainhdl.cpp:1049-1141), a sequence of OpenMP phase loops that call one LSbox method
+ * per grain (src/box.h:124-191).  libggx replaces that loop body.  The retained driver
+ * (settings, microstructure input, output writers) stays on the host; one ggx_step() call
+ * performs what the phase loops switchDistancebuffer / convolution / updateSecondOrderNeighbors /
+ * comparison_box / level_set / redistancing do for every grain, on the GPU.
+ *
+ * Conventions: plain C types only; every function returns a status code (GGX_OK == 0 or a
+ * negative GGX_E_*); no exceptions cross the ABI; host buffers passed in are caller-owned and
+ * copied; ggx_last_error() describes the last failure of a context.
+ */
+#ifndef GGX_H
+#define GGX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GGX_OK        0
+#define GGX_E_CONFIG (-1)   /* invalid settings (reference: exit(2), Settings.cpp:88-108) */
+#define GGX_E_HIP    (-2)   /* a HIP runtime call failed */
+#define GGX_E_OOM    (-3)   /* device arena or host allocation exhausted */
+#define GGX_E_BOUNDS (-4)   /* an index left its buffer (reference: std::out_of_range) */
+#define GGX_E_STATE  (-5)   /* call out of order (e.g. step before load) */
+#define GGX_E_ARG    (-6)   /* bad argument */
+#define GGX_E_IO     (-7)   /* file could not be read or written */
+
+/* ---------------------------------------------------------------------------------------
+ * Settings: the parameters.xml keys the timestep reads (Settings.h:69-121,
+ * Settings::initializeParameters, src/Settings.cpp:84-289).
+ * ------------------------------------------------------------------------------------- */
+typedef struct ggx_settings {
+    int32_t start_time;                /* StartTime */
+    int32_t number_of_particles;       /* NumberOfParticles */
+    int32_t number_of_points_per_grain;/* NumberOfPointsPerGrain */
+    int32_t number_of_timesteps;       /* NumberOfTimesteps */
+    int32_t breakup_number;            /* BreakupNumber */
+    int32_t analysis_timestep;         /* AnalysisTimestep */
+    int32_t plot_interval;             /* PlotInterval */
+    int32_t discrete_sampling_rate;    /* DiscreteSamplingRate */
+    int32_t domain_border_size;        /* DomainBorderSize (grid_blowup) */
+    int32_t microstructure_gen_mode;   /* MicrostructureGenMode (0 = read from file) */
+    int32_t lattice_type;              /* LatticeType (0 cubic) */
+    int32_t use_mobility_model;        /* UseMobilityModel */
+    int32_t identify_twins;            /* IdentifyTwins */
+    int32_t use_magnetic_field;        /* UseMagneticField */
+    int32_t is_isotropic_network;      /* IsIsotropicNetwork */
+    int32_t use_texture;               /* UseTexture */
+    int32_t execute_in_parallel;       /* ExecuteInParallel */
+    int32_t maximum_number_of_threads; /* MaximumNumberOfThreads */
+    int32_t grid_coarsement;           /* GridCoarsement */
+    int32_t convolution_mode;          /* ConvolutionMode: 0 Laplace, 1 Richardson, 2 Gaussian */
+    int32_t research_mode;             /* ResearchMode */
+    int32_t research_project;          /* ResearchProject */
+    int32_t neighbour_tracking;        /* NeighbourTracking */
+    int32_t use_stored_elastic_energy; /* UseStoredElasticEnergy */
+    int32_t grain_scheduler;           /* GrainScheduler */
+    int32_t decouple_grains;           /* DecoupleGrains */
+    int32_t user_def_number_of_points_per_grain; /* UserDefNumberOfPointsPerGrain */
+    double  hagb_mobility;             /* HAGB_Mobility */
+    double  hagb_energy;               /* HAGB_Energy */
+    double  physical_domain_size;      /* Physical_Domain_Size */
+    double  triple_point_drag;         /* TriplePointDrag */
+    double  disloc_en_per_m;           /* DislocEnPerM */
+    double  max_misorientation;        /* MaxMisOrientation */
+    double  grid_coarsement_gradient;  /* GridCoarsementGradient */
+    double  constant_sector_radius;    /* ConstantSectorRadius (grid points, as read) */
+    double  interpolating_sector_radius; /* InterpolatingSectorRadius (grid points, as read) */
+    double  gaussian_kernel_time_slope;/* GaussianKernelUserDefTimeSlope */
+    double  box_default_stored_elastic_energy; /* BoxDefaultStoredElasticEnergy */
+    char    read_from_filename[512];   /* ReadFromFilename */
+    char    additional_filename[512];  /* AdditionalFilename */
+} ggx_settings;
+
+/* Reference defaults of the static Settings fields (src/Settings.cpp:29-80). */
+int ggx_settings_default(ggx_settings* s);
+/* Settings::initializeParameters (src/Settings.cpp:84-289): same keys, same parsing. */
+int ggx_settings_read_xml(const char* path, ggx_settings* s);
+/* Settings::generateXMLParametersNode + grainhdl::createParamsForSim
+ * (src/Settings.cpp:363-400, src/grainhdl.cpp:1205-1223). */
+int ggx_settings_write_xml(const char* path, const ggx_settings* s, const char* network_file,
+                           int32_t loop, int32_t grains);
+
+/* Derived simulation constants, grainhdl::setSimulationParameter (src/grainhdl.cpp:56-136)
+ * and find_correctTimestepSize (250-274) for MicrostructureGenMode 0. */
+typedef struct ggx_derived {
+    int32_t real_domain_size;  /* R */
+    int32_t ngridpoints;       /* R + 2 grid_blowup */
+    int32_t grid_blowup;
+    double  h, dt, delta, time_slope, ds;
+} ggx_derived;
+int ggx_settings_derive(const ggx_settings* s, ggx_derived* d);
+
+/* ---------------------------------------------------------------------------------------
+ * Microstructure: the mode-0 network file (grainhdl::readMicrostructure,
+ * src/grainhdl.cpp:605-661; writer LSbox::plot_full_grain, src/box.cpp:1552-1596).
+ * ------------------------------------------------------------------------------------- */
+typedef struct ggx_network ggx_network;
+
+/* Seeded microstructure: Voronoi (lognormal_sigma == 0) or power diagram with lognormal
+ * weights, uniform seed points in [0,1)^2 and Shoemake-uniform orientations.  Stands in for
+ * grainhdl::VOROMicrostructure (src/grainhdl.cpp:513-603), whose rand()/voro++ path is not
+ * reproducible.  Polygons are closed (first vertex repeated last). */
+int  ggx_network_generate(int32_t n, uint64_t seed, double lognormal_sigma, ggx_network** out);
+int  ggx_network_read(const char* path, int32_t ngrains, ggx_network** out);
+int  ggx_network_write(const char* path, const ggx_network* net);
+int  ggx_network_count(const ggx_network* net);
+/* View of grain `idx` (0-based): its ID, vertex count and pointers valid until the network is
+ * freed. */
+int  ggx_network_grain(const ggx_network* net, int32_t idx, int32_t* id, int32_t* nverts,
+                       const double** xy, const double** quat);
+int  ggx_network_add(ggx_network* net, int32_t id, int32_t nverts, const double* xy,
+                     const double* quat);
+int  ggx_network_new(ggx_network** out);
+void ggx_network_free(ggx_network* net);
+
+/* ---------------------------------------------------------------------------------------
+ * Device context.
+ * ------------------------------------------------------------------------------------- */
+typedef struct ggx_ctx ggx_ctx;
+
+#define GGX_PREC_FP64 0
+#define GGX_PREC_FP32 1
+
+#define GGX_BATCH_FUSED    0  /* all grains per launch (the batched step) */
+#define GGX_BATCH_PERGRAIN 1  /* one launch per grain per kernel (the per-grain baseline) */
+
+#define GGX_CONV_AUTO    0    /* autotuned choice per box size */
+#define GGX_CONV_STENCIL 1    /* exact banded circulant (real space, separable) */
+#define GGX_CONV_FFT     2    /* LDS-staged mixed-radix FFT */
+
+typedef struct ggx_options {
+    int32_t precision;        /* GGX_PREC_* */
+    int32_t batching;         /* GGX_BATCH_* */
+    int32_t conv_engine;      /* GGX_CONV_* */
+    int32_t device;           /* HIP device ordinal */
+    int32_t lds_budget;       /* bytes of LDS a workgroup may use; 0 = device maximum */
+    int32_t use_graph;        /* capture the step in a hipGraph */
+    double  arena_headroom;   /* arena capacity = headroom * initial sum of box areas */
+} ggx_options;
+int ggx_options_default(ggx_options* o);
+
+/* Replaces grainhdl::setSimulationParameter's context set-up (grainhdl.cpp:56-136). */
+int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx** out);
+void ggx_destroy(ggx_ctx* ctx);
+const char* ggx_last_error(const ggx_ctx* ctx);
+
+/* Builds every grain (LSbox file constructor, box.cpp:108-158), its initial signed distance
+ * (LSbox::calculateDistanceFunction, box.cpp:318-402) and the initial neighbour lists
+ * (grainhdl::find_neighbors, grainhdl.cpp:1258-1283), then uploads them to the device arena. */
+int ggx_load_network(ggx_ctx* ctx, const ggx_network* net);
+
+/* Runs `nsteps` passes of the run_sim loop body (grainhdl.cpp:1049-1141): per grain
+ * LSbox::executeConvolution (box.cpp:408-536), computeSecondOrderNeighbours (1027-1068),
+ * executeComparison (890-943), executeSetComparison (838-879), extractContour (1089-1153),
+ * computeVolumeAndEnergy (1201-1228), executeRedistancing (1259-1399), with the buffer
+ * switches of switchInNOut (826-832).  The loop index advances by one per step. */
+int ggx_step(ggx_ctx* ctx, int32_t nsteps);
+/* Blocks until queued device work is done; returns the first deferred device error. */
+int ggx_sync(ggx_ctx* ctx);
+/* Current loop index (the next step runs with this `loop`). */
+int ggx_loop(const ggx_ctx* ctx);
+/* grainhdl::get_biggestGrainVol currentNrGrains (grainhdl.cpp:1744-1754). */
+int ggx_current_grains(ggx_ctx* ctx);
+int ggx_max_id(const ggx_ctx* ctx);
+
+/* Per-grain state after the last step (LSbox fields, box.h:71-96). */
+typedef struct ggx_grain_info {
+    int32_t present;              /* grains[id] != NULL */
+    int32_t exists;               /* LSbox::grainExists() */
+    int32_t motion_regular;       /* LSbox::isMotionRegular() */
+    int32_t intersects_boundary;  /* LSbox::intersectsBoundaryGrain() */
+    int32_t xmin, xmax, ymin, ymax;             /* getMinX()..getMaxY() (output buffer) */
+    int32_t id_xmin, id_xmax, id_ymin, id_ymax; /* bounds of m_IDLocal */
+    int32_t contour_len;          /* m_grainBoundary.getBoundarySegmentCount() */
+    int32_t n_neighbours;         /* getDirectNeighbourCount() */
+    int32_t n_compare;            /* comparison-list length */
+    int32_t error;                /* non-zero: the reference would have aborted */
+    double  volume, perimeter, energy, mean_da;  /* getVolume(), getPerimeter(), ... */
+} ggx_grain_info;
+
+int ggx_get_info(ggx_ctx* ctx, int32_t id, ggx_grain_info* out);
+/* Bulk copy of every grain's info, out[id] for id in [0, max_id] (out[0] is the boundary). */
+int ggx_get_info_all(ggx_ctx* ctx, ggx_grain_info* out, int32_t count);
+/* Signed distance of grain `id` on its box, row-major (row = y), widened to double for the
+ * FP32 path.  Returns the element count or a negative status. */
+int ggx_get_phi(ggx_ctx* ctx, int32_t id, double* out, int32_t cap);
+/* m_IDLocal of grain `id` (its full flat vector). */
+int ggx_get_idlocal(ggx_ctx* ctx, int32_t id, uint32_t* out, int32_t cap);
+/* Ordered contour (x, y pairs, grid-index units) -- m_grainBoundary.getRawBoundary(). */
+int ggx_get_contour(ggx_ctx* ctx, int32_t id, double* xy, int32_t cap_points);
+/* getDirectNeighbourIDs() / getGBLengths() (box.cpp:2212-2218), first-encounter order. */
+int ggx_get_neighbours(ggx_ctx* ctx, int32_t id, int32_t* ids, double* lengths, int32_t cap);
+/* Comparison list in list order (LSbox::m_comparisonList). */
+int ggx_get_compare_list(ggx_ctx* ctx, int32_t id, int32_t* ids, int32_t cap);
+
+/* Per-phase timers in the reference's stdout categories (grainhdl.cpp:1144-1154), in ms,
+ * accumulated over all steps, plus the work and traffic counters of the last step. */
+typedef struct ggx_stats {
+    double  convolution_ms, comparison_ms, levelset_ms, redistancing_ms, rest_ms, total_ms;
+    int64_t steps;
+    int64_t grid_point_updates;   /* sum over grains of n_g^2 in the last step */
+    int64_t algorithmic_bytes;    /* B_step of the last step */
+    int64_t conv_bytes;           /* bytes the convolution kernel must move in the last step */
+    int32_t live_grains;
+    int32_t max_box;
+} ggx_stats;
+int ggx_get_stats(ggx_ctx* ctx, ggx_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,986 @@
+/*
+ * ggx_oracle.cpp -- CPU oracle (TEST INFRASTRUCTURE) for GraGLeS2D's per-grain level-set
+ * timestep, isotropic network, FP64.
+ *
+ * This file restates, function by function, what the reference computes on the hot path.
+ * Each function cites the reference file:line it follows.  It is compiled with g++/OpenMP
+ * (oracle/Makefile) into oracle/libggx_oracle.so and is loaded only by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ *
+ * Pinning (see tests/test_oracle.py):
+ *   - ggxo_dft / ggxo_convolve: numpy FFT golden vectors (tests/golden/conv_*.npz), a
+ *     single-cosine-mode known-answer test, DC gain, symmetry.
+ *   - comparison / set-comparison / boundary condition: brute-force numpy restatement on
+ *     small boxes and the two-grain antisymmetry invariant.
+ *   - redistancing: half-plane fixed point, sign preservation, |phi| <= delta.
+ *   - marching squares: square / circle areas and perimeters, 16 corner masks.
+ *   - whole step: area conservation, isolated-circle shrink rate, neighbour symmetry.
+ * Known deviation: the initial comparison lists are in ascending grain-ID order, where the
+ * reference takes the traversal order of its R-tree (src/RTree.h via grainhdl.cpp:1258-1283);
+ * the order only decides exact ties in the comparison maximum.
+ */
+#include "ggx_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kPi = 3.14159265358979323846;  // ggLS.h:29 (PI)
+
+/* ------------------------------------------------------------------------------------
+ * Buffers.  DimensionalBuffer<T> (src/dimensionalBuffer.h:34-211): a window [xmin,xmax) x
+ * [ymin,ymax) in global grid coordinates over a flat vector; resize() keeps the vector prefix
+ * and zero-fills growth (std::vector::resize); getValueAt() computes the flat index in 32-bit
+ * unsigned arithmetic and bounds-checks it against the vector size only (lines 68-73).
+ * ------------------------------------------------------------------------------------ */
+template <class T>
+struct DBuf {
+    int xmin = 0, xmax = 1, ymin = 0, ymax = 1;
+    std::vector<T> v;
+
+    int w() const { return xmax - xmin; }
+    int hgt() const { return ymax - ymin; }
+    void resize(int x0, int y0, int x1, int y1) {  // dimensionalBuffer.h:108-115
+        xmin = x0; xmax = x1; ymin = y0; ymax = y1;
+        long long s = (long long)(xmax - xmin) * (long long)(ymax - ymin);
+        if (s < 0) s = 0;
+        v.resize((size_t)s);
+    }
+    bool flat(int row, int col, size_t& k) const {  // dimensionalBuffer.h:71-72
+        uint32_t kk = (uint32_t)row - (uint32_t)ymin;
+        kk *= (uint32_t)(xmax - xmin);
+        kk += (uint32_t)col - (uint32_t)xmin;
+        if ((size_t)kk >= v.size()) return false;
+        k = kk;
+        return true;
+    }
+    T& at(int row, int col) {
+        size_t k;
+        if (!flat(row, col, k)) throw std::out_of_range("DimensionalBuffer::getValueAt");
+        return v[k];
+    }
+    // direct access for loops whose bounds keep the index inside the window
+    T& d(int row, int col) { return v[(size_t)(row - ymin) * (size_t)(xmax - xmin) + (size_t)(col - xmin)]; }
+    void fill(const T& val) { std::fill(v.begin(), v.end(), val); }  // clearValues (149-166)
+
+    void resizeToSquare(int maximumLength) {  // dimensionalBuffer.h:123-160
+        int grid_size = maximumLength - 1;
+        unsigned int height = (unsigned int)(ymax - ymin);
+        unsigned int width = (unsigned int)(xmax - xmin);
+        if (width == height) return;
+        if (width > height) {
+            int diff = (int)(width - height);
+            ymin -= diff / 2;
+            ymax += diff / 2 + diff % 2;
+        } else {
+            int diff = (int)(height - width);
+            xmin -= diff / 2;
+            xmax += diff / 2 + diff % 2;
+        }
+        if (xmin < 0) { int dl = -xmin; xmax += dl; xmin += dl; }
+        else if (xmax > grid_size) { int dl = grid_size - xmax; xmax += dl; xmin += dl; }
+        if (ymin < 0) { int dl = -ymin; ymax += dl; ymin += dl; }
+        else if (ymax > grid_size) { int dl = grid_size - ymax; ymax += dl; ymin += dl; }
+        resize(xmin, ymin, xmax, ymax);
+    }
+};
+
+struct Pt {  // SPoint (src/spoint.h:29-87)
+    double x = -1, y = -1, energy = 0, mob = 1;
+    Pt() {}
+    Pt(double x_, double y_, double e_, double m_) : x(x_), y(y_), energy(e_), mob(m_) {}
+};
+inline double seglen(const Pt& a, const Pt& b) {  // (a - b).len()
+    double dx = a.x - b.x, dy = a.y - b.y;
+    return std::sqrt(dx * dx + dy * dy);
+}
+
+struct Neighbour {  // characteristics (src/charasteristic.h:29-62)
+    int id;
+    double length;
+};
+
+template <class T>
+inline int sgn(T v) { return (T(0) < v) - (v < T(0)); }  // utilities.h:27-30
+
+/* ------------------------------------------------------------------------------------
+ * FFT: mixed-radix recursive Cooley-Tukey on complex data, unnormalised, FFTW sign convention
+ * (forward e^{-2 pi i jk/n}).  Stands in for the FFTW r2c/c2r plans of box.cpp:580-608.
+ * ------------------------------------------------------------------------------------ */
+struct Cplx { double re, im; };
+
+struct FftPlan {
+    int n = 0;
+    std::vector<double> c, s;  // cos/sin(2 pi e / n), e in [0,n)
+    explicit FftPlan(int n_) : n(n_), c(n_), s(n_) {
+        for (int e = 0; e < n; ++e) {
+            double a = 2.0 * kPi * (double)e / (double)n;
+            c[e] = std::cos(a);
+            s[e] = std::sin(a);
+        }
+    }
+};
+
+int smallest_factor(int n) {
+    if (n % 4 == 0 && n > 4) return 4;
+    for (int p = 2; p * p <= n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+// out[k] = sum_j in[j*stride] * w^{jk}, w = e^{sign 2 pi i / n}; `n` divides plan.n.
+void fft_rec(const Cplx* in, int stride, Cplx* out, int n, int sign, const FftPlan& pl,
+             std::vector<Cplx>& scratch, size_t sofs) {
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = smallest_factor(n);
+    int m = n / p;
+    int tstep = pl.n / n;  // twiddle index scale: w_n^e = w_N^{e*tstep}
+    // sub-transforms Y_r[k] = FFT_m(in[r + p*j]) stored at out[r*m + k]
+    for (int r = 0; r < p; ++r) fft_rec(in + (size_t)r * stride, stride * p, out + (size_t)r * m, m, sign, pl, scratch, sofs + (size_t)n);
+    if (scratch.size() < sofs + (size_t)n) scratch.resize(sofs + (size_t)n);
+    Cplx* tmp = scratch.data() + sofs;
+    for (int k = 0; k < m; ++k) {
+        for (int q = 0; q < p; ++q) {
+            int kk = k + q * m;
+            double sr = 0.0, si = 0.0;
+            for (int r = 0; r < p; ++r) {
+                long long e = ((long long)r * kk) % n;
+                int ei = (int)(e * tstep);
+                double wr = pl.c[ei], wi = sign * pl.s[ei];
+                const Cplx& y = out[(size_t)r * m + k];
+                sr += y.re * wr - y.im * wi;
+                si += y.re * wi + y.im * wr;
+            }
+            tmp[kk].re = sr;
+            tmp[kk].im = si;
+        }
+    }
+    for (int k = 0; k < n; ++k) out[k] = tmp[k];
+}
+
+void fft1d(Cplx* data, int n, int stride, int sign, const FftPlan& pl, std::vector<Cplx>& work,
+           std::vector<Cplx>& scratch) {
+    work.resize((size_t)n);
+    std::vector<Cplx> in((size_t)n);
+    for (int k = 0; k < n; ++k) in[k] = data[(size_t)k * stride];
+    fft_rec(in.data(), 1, work.data(), n, sign, pl, scratch, 0);
+    for (int k = 0; k < n; ++k) data[(size_t)k * stride] = work[k];
+}
+
+// convolutionGeneratorFFTW (box.cpp:622-695): out = IDFT( G . DFT(in) ), G includes 1/n^2.
+void convolve_box(int n, int mode, double dt, int R, const double* in, double* out) {
+    FftPlan pl(n);
+    std::vector<Cplx> a((size_t)n * n), work, scratch;
+    for (size_t k = 0; k < (size_t)n * n; ++k) { a[k].re = in[k]; a[k].im = 0.0; }
+    for (int r = 0; r < n; ++r) fft1d(a.data() + (size_t)r * n, n, 1, -1, pl, work, scratch);
+    for (int c = 0; c < n; ++c) fft1d(a.data() + c, n, n, -1, pl, work, scratch);
+    int nn = R;                       // box.cpp:636-637: nsq uses the GLOBAL grid size
+    double nsq = (double)nn * (double)nn;
+    double kk = 2.0 * kPi / n;
+    double n_nsq = (double)(n * n);   // box.cpp:675
+    for (int j = 0; j < n; ++j) {     // j: y frequency (row index of the spectrum)
+        for (int i = 0; i < n; ++i) { // i: x frequency
+            double G = 0.0;
+            if (mode == 2) {          // E_GAUSSIAN, box.cpp:674-688
+                int i2 = std::min(i, n - i);
+                int j2 = std::min(j, n - j);
+                G = std::exp(-std::sqrt(2 / kPi) * (i2 * i2 + j2 * j2) * 4.0 * dt * nsq / n_nsq * kPi * kPi) / n_nsq;
+            } else if (mode == 0) {   // E_LAPLACE, box.cpp:648-659
+                double coski = std::cos(kk * i);
+                G = 2.0 * (2.0 - coski - std::cos(kk * j)) * nsq;
+                G = 1.0 / (1.0 + (dt * G)) / (n * n);
+            } else if (mode == 1) {   // E_LAPLACE_RITCHARDSON, box.cpp:660-673
+                double coski = std::cos(kk * i);
+                G = 2.0 * (2.0 - coski - std::cos(kk * j)) * nsq;
+                G = (4.0 / std::pow(1 + 1.5 * (dt) / 40 * G, 40) - 1.0 / std::pow(1 + 3.0 * (dt) / 40 * G, 40)) / 3.0 / (double)(n * n);
+            } else {
+                G = 1.0;  // default branch (box.cpp:689-690): no multiply, unnormalised c2r
+            }
+            Cplx& z = a[(size_t)j * n + i];
+            z.re *= G;
+            z.im *= G;
+        }
+    }
+    for (int c = 0; c < n; ++c) fft1d(a.data() + c, n, n, +1, pl, work, scratch);
+    for (int r = 0; r < n; ++r) fft1d(a.data() + (size_t)r * n, n, 1, +1, pl, work, scratch);
+    for (size_t k = 0; k < (size_t)n * n; ++k) out[k] = a[k].re;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Grain state (LSbox, src/box.h:68-111)
+ * ------------------------------------------------------------------------------------ */
+struct Grain {
+    int id = 0;
+    bool present = false;  // grains[id] != NULL
+    bool exists = true;
+    bool motion_regular = true;
+    bool intersects_boundary = false;
+    int error = 0;
+    double q[4] = {1, 0, 0, 0};
+    DBuf<double>* in = nullptr;
+    DBuf<double>* out = nullptr;
+    DBuf<double> bufA, bufB;
+    DBuf<uint32_t> idl;  // m_IDLocal
+    std::vector<Pt> contour;  // m_grainBoundary.m_grainBoundary
+    std::vector<Neighbour> neigh;  // m_directNeighbourhood
+    std::vector<int> cmp;  // m_comparisonList
+    std::vector<int> second;  // m_secondOrderNeighbours
+    double volume = 0, energy = 0, perimeter = 0, meanDa = 0;
+    int newXMin = 0, newXMax = 0, newYMin = 0, newYMax = 0;
+};
+
+struct Ctx {
+    ggxo_params p;
+    int max_id = 0;
+    int R = 0, gb = 0, m = 0;
+    double h = 0, delta = 0, dt = 0;
+    std::vector<Grain> g;  // index = id, g[0] is the domain-boundary grain
+    int loop = 0;
+};
+
+/* LSbox file constructor (box.cpp:108-158): box from the polygon, then resizeToSquare. */
+void construct_grain(Ctx& C, Grain& G, const std::vector<Pt>& verts) {
+    G.contour = verts;
+    int xmax = 0, xmin = C.m, ymax = 0, ymin = xmin;
+    double h = C.h;
+    for (size_t k = 0; k < verts.size(); ++k) {
+        double y = verts[k].y, x = verts[k].x;
+        if (y / h < ymin) ymin = (int)(y / h);
+        if (y / h > ymax) ymax = (int)(y / h);
+        if (x / h < xmin) xmin = (int)(x / h);
+        if (x / h > xmax) xmax = (int)(x / h);
+    }
+    xmax += 2 * C.gb;
+    ymax += 2 * C.gb;
+    if (ymax > C.m) ymax = C.m;
+    if (xmax > C.m) xmax = C.m;
+    G.bufA.resize(xmin, ymin, xmax, ymax);
+    G.bufB.resize(xmin, ymin, xmax, ymax);
+    G.in = &G.bufA;
+    G.out = &G.bufB;
+    G.in->resizeToSquare(C.m);
+    G.out->resizeToSquare(C.m);
+    G.idl.resize(G.out->xmin, G.out->ymin, G.out->xmax, G.out->ymax);  // box.cpp:571-578
+}
+
+/* LSbox::computeVolume -> ExplicitGrainBoundary::computeVolume (grainBoundary.cpp:201-213) */
+double contour_volume(const std::vector<Pt>& c, double h) {
+    double v = 0;
+    for (size_t i = 0; i + 1 < c.size(); ++i) v += (c[i].y + c[i + 1].y) * (c[i].x - c[i + 1].x);
+    return std::fabs(v) * h * h * 0.5;
+}
+
+/* LSbox::calculateDistanceFunction, MicrostructureGenMode != voxel (box.cpp:342-401) */
+void distance_function(Ctx& C, Grain& G) {
+    int gb = C.gb;
+    double h = C.h;
+    const std::vector<Pt>& cg = G.contour;
+    int cs = (int)cg.size();
+    DBuf<double>& o = *G.out;
+    for (int i = o.ymin; i < o.ymax; ++i) {
+        for (int j = o.xmin; j < o.xmax; ++j) {
+            double tx = (j - gb) * h, ty = (i - gb) * h;
+            bool inside = false;
+            for (int k = 1, l = 0; k < cs; k++) {
+                if (((cg[l].y > ty) != (cg[k].y > ty)) &&
+                    (tx < (cg[k].x - cg[l].x) * (ty - cg[l].y) / (cg[k].y - cg[l].y) + cg[l].x))
+                    inside = !inside;
+                l = k;
+            }
+            double minDist = 1000000.0;
+            for (int k = 1, l = 0; k < cs; k++) {
+                double ux = cg[k].x - cg[l].x, uy = cg[k].y - cg[l].y;
+                double lambda = (tx - cg[l].x) * ux + (ty - cg[l].y) * uy;
+                lambda /= ux * ux + uy * uy;
+                double dist;
+                if (lambda <= 0) {
+                    double dx = tx - cg[l].x, dy = ty - cg[l].y;
+                    dist = std::sqrt(dx * dx + dy * dy);
+                } else if (lambda >= 1) {
+                    double dx = cg[k].x - tx, dy = cg[k].y - ty;
+                    dist = std::sqrt(dx * dx + dy * dy);
+                } else {
+                    // SPoint::operator* returns x,y only; (l + u*lambda)
+                    double px = cg[l].x + ux * lambda, py = cg[l].y + uy * lambda;
+                    double dx = tx - px, dy = ty - py;
+                    dist = std::sqrt(dx * dx + dy * dy);
+                }
+                minDist = std::min(minDist, dist);
+                l = k;
+            }
+            if (minDist > C.delta) minDist = C.delta;
+            o.d(i, j) = inside ? minDist : -minDist;
+        }
+    }
+    G.volume = contour_volume(cg, h) / (h * h);
+}
+
+/* closed-interval box test on the input buffers, LSbox::checkIntersection (box.cpp:881-888) */
+bool check_intersection(const Grain& a, const Grain& b) {
+    if (a.in->xmin > b.in->xmax || a.in->xmax < b.in->xmin || a.in->ymin > b.in->ymax || a.in->ymax < b.in->ymin)
+        return false;
+    return true;
+}
+
+/* LSbox::computeSecondOrderNeighbours (box.cpp:1027-1068) */
+void second_order_neighbours(Ctx& C, Grain& G) {
+    if (!G.exists) return;
+    std::vector<int> cand;
+    G.cmp.clear();
+    for (int nb : G.second) {
+        Grain& N = C.g[nb];
+        if (N.exists) G.cmp.push_back(nb);
+        for (int nn : N.second) {
+            Grain& M = C.g[nn];
+            if (M.exists && check_intersection(G, M)) cand.push_back(nn);
+        }
+    }
+    for (int c : cand) {
+        if (c == G.id) continue;
+        if (c == 0) continue;
+        bool just_in = false;
+        for (int x : G.cmp)
+            if (x == c) { just_in = true; break; }
+        if (!just_in) G.cmp.push_back(c);
+    }
+}
+
+/* LSbox::BoundaryIntersection (box.cpp:945-962) */
+bool boundary_intersection(const Ctx& C, const Grain& G) {
+    int lo = C.gb + C.gb;  // grid_blowup + BoundaryGrainTube (tube = grid_blowup, grainhdl.cpp:135)
+    int hi = C.m - C.gb - C.gb;
+    const DBuf<double>& o = *G.out;
+    if (o.xmin > lo && o.xmax < hi && o.ymin > lo && o.ymax < hi) return false;
+    return true;
+}
+
+/* LSbox::boundaryCondition (box.cpp:968-1025) */
+void boundary_condition(Ctx& C, Grain& G) {
+    int gb = C.gb, m = C.m;
+    double h = C.h;
+    double dist = 0;
+    DBuf<double>& in = *G.in;
+    DBuf<double>& o = *G.out;
+    for (int i = in.ymin; i < in.ymax; i++) {
+        for (int j = in.xmin; j < in.xmax; j++) {
+            int distXMin = -(j - gb), distYMin = -(i - gb);
+            int distXMax = (j - (m - gb)), distYMax = (i - (m - gb));
+            int distX = (std::abs(distXMin) < std::abs(distXMax)) ? distXMin : distXMax;
+            int distY = (std::abs(distYMin) < std::abs(distYMax)) ? distYMin : distYMax;
+            if (distX > 0 && distY > 0) dist = std::sqrt((double)distX * distX + distY * distY);
+            else if (distX < 0 && distY > 0) dist = distY;
+            else if (distX > 0 && distY < 0) dist = distX;
+            else if (distX < 0 && distY < 0) dist = std::max(distX, distY);
+            else if (distX == 0) {
+                if (distY == 0) dist = 0;
+                else if (distY < 0) dist = 0;
+                else if (distY > 0) dist = distY;
+            } else if (distY == 0) {
+                if (distX < 0) dist = 0;
+                else if (distX > 0) dist = distX;
+            }
+            if (dist * h > o.d(i, j)) {
+                o.d(i, j) = dist * h;
+                G.idl.at(i, j) = 0;
+            }
+        }
+    }
+}
+
+/* LSbox::executeComparison (box.cpp:890-943) */
+void execute_comparison(Ctx& C, Grain& G) {
+    if (!G.exists) return;
+    DBuf<double>& in = *G.in;
+    DBuf<double>& o = *G.out;
+    o.fill(-1.0);
+    G.second = G.cmp;
+    for (int nb : G.second) {
+        const DBuf<double>& nin = *C.g[nb].in;
+        int x0 = in.xmin < nin.xmin ? nin.xmin : in.xmin;
+        int x1 = in.xmax > nin.xmax ? nin.xmax : in.xmax;
+        int y0 = in.ymin < nin.ymin ? nin.ymin : in.ymin;
+        int y1 = in.ymax > nin.ymax ? nin.ymax : in.ymax;
+        for (int i = y0; i < y1; i++) {
+            for (int j = x0; j < x1; j++) {
+                double dist = const_cast<DBuf<double>&>(nin).d(i, j);
+                if (dist > o.d(i, j)) {
+                    o.d(i, j) = dist;
+                    G.idl.at(i, j) = (uint32_t)nb;
+                }
+            }
+        }
+    }
+    if (boundary_intersection(C, G)) {
+        G.intersects_boundary = true;
+        boundary_condition(C, G);
+    } else {
+        G.intersects_boundary = false;
+    }
+}
+
+/* LSbox::executeSetComparison (box.cpp:838-879) */
+void execute_set_comparison(Ctx& C, Grain& G) {
+    DBuf<double>& in = *G.in;
+    DBuf<double>& o = *G.out;
+    G.newXMin = o.xmax; G.newXMax = o.xmin;
+    G.newYMin = o.ymax; G.newYMax = o.ymin;
+    double band = 0.7 * C.delta;
+    for (int i = o.ymin; i < o.ymax; i++) {
+        for (int j = o.xmin; j < o.xmax; j++) {
+            double vin = in.d(i, j);
+            if (std::fabs(vin) < band) o.d(i, j) = 0.5 * (vin - o.d(i, j));
+            else o.d(i, j) = vin;
+            if (o.d(i, j) >= 0) {
+                if (i < G.newYMin) G.newYMin = i;
+                if (i > G.newYMax) G.newYMax = i;
+                if (j < G.newXMin) G.newXMin = j;
+                if (j > G.newXMax) G.newXMax = j;
+            }
+        }
+    }
+    G.newXMin -= C.gb; G.newXMax += C.gb;
+    G.newYMin -= C.gb; G.newYMax += C.gb;
+    if (G.newXMin >= G.newXMax || G.newYMin >= G.newYMax) {
+        G.newXMin = o.xmin; G.newXMax = o.xmax;
+        G.newYMin = o.ymin; G.newYMax = o.ymax;
+    }
+}
+
+/* MarchingSquaresAlgorithm (marchingSquares.cpp:28-301).  Returns false when no inside point
+ * exists.  Junction detection (generateJunction) only feeds the anisotropic sectors and the
+ * output statistics; the isotropic step does not need it. */
+enum { TOP_LEFT = 1, TOP_RIGHT = 2, BOTTOM_LEFT = 4, BOTTOM_RIGHT = 8 };
+enum Dir { UP, DOWN, LEFT, RIGHT, NO_DIRECTION };
+
+struct March {
+    DBuf<double>& D;
+    int top = -1, bottom = -1, left = -1, right = -1;
+    explicit March(DBuf<double>& d) : D(d) {}
+    bool inside(int row, int col) {  // marchingSquares.cpp:189-196
+        if (row < D.ymin || row >= D.ymax || col < D.xmin || col >= D.xmax) return false;
+        return D.d(row, col) >= 0;
+    }
+    Pt point(Dir dir) {  // marchingSquares.cpp:204-265
+        Pt r(-1, -1, -1, -1);
+        double delta, position;
+        switch (dir) {
+            case LEFT:
+                r.x = left;
+                if (top < D.ymin) r.y = bottom;
+                else {
+                    delta = D.d(bottom, left) - D.d(top, left);
+                    position = bottom - D.d(bottom, left) / delta;
+                    r.y = position;
+                }
+                break;
+            case RIGHT:
+                r.x = right;
+                if (bottom >= D.ymax) r.y = top;
+                else {
+                    delta = D.d(top, right) - D.d(bottom, right);
+                    position = top + D.d(top, right) / delta;
+                    r.y = position;
+                }
+                break;
+            case UP:
+                r.y = top;
+                if (right >= D.xmax) r.x = left;
+                else {
+                    delta = D.d(top, left) - D.d(top, right);
+                    position = left + D.d(top, left) / delta;
+                    r.x = position;
+                }
+                break;
+            case DOWN:
+                r.y = bottom;
+                if (left < D.xmin) r.x = right;
+                else {
+                    delta = D.d(bottom, right) - D.d(bottom, left);
+                    position = right - D.d(bottom, right) / delta;
+                    r.x = position;
+                }
+                break;
+            default: break;
+        }
+        return r;
+    }
+    static void insert(std::vector<Pt>& o, const Pt& p) {  // marchingSquares.cpp:197-203
+        if (o.empty() || p.x != o.back().x || p.y != o.back().y) o.push_back(p);
+    }
+    // returns 0 no contour, 1 ok, -1 iteration cap hit (the reference would loop forever)
+    int contour(std::vector<Pt>& out) {
+        out.clear();
+        int sx = -1, sy = -1;
+        for (int i = D.ymin, done = 0; i < D.ymax && !done; i++)
+            for (int j = D.xmin; j < D.xmax; j++)
+                if (inside(i, j)) { sx = j; sy = i; done = 1; break; }
+        if (sx == sy && sx == -1) return 0;
+        left = sx - 1; right = sx; top = sy - 1; bottom = sy;
+        Dir next = NO_DIRECTION, prev = NO_DIRECTION;
+        long long cap = 8LL * (long long)(D.w() + 2) * (long long)(D.hgt() + 2) + 16;
+        long long it = 0;
+        do {
+            int mask = 0;
+            if (inside(top, left)) mask |= TOP_LEFT;
+            if (inside(top, right)) mask |= TOP_RIGHT;
+            if (inside(bottom, left)) mask |= BOTTOM_LEFT;
+            if (inside(bottom, right)) mask |= BOTTOM_RIGHT;
+            switch (mask) {
+                case TOP_LEFT: next = UP; insert(out, point(next)); break;
+                case TOP_RIGHT: next = RIGHT; insert(out, point(next)); break;
+                case TOP_LEFT | TOP_RIGHT: next = RIGHT; insert(out, point(next)); break;
+                case BOTTOM_LEFT: next = LEFT; insert(out, point(next)); break;
+                case BOTTOM_LEFT | TOP_LEFT: next = UP; insert(out, point(next)); break;
+                case BOTTOM_LEFT | TOP_RIGHT: next = (prev == UP) ? RIGHT : LEFT; break;
+                case BOTTOM_LEFT | TOP_RIGHT | TOP_LEFT: next = RIGHT; insert(out, point(next)); break;
+                case BOTTOM_RIGHT: next = DOWN; insert(out, point(next)); break;
+                case BOTTOM_RIGHT | TOP_LEFT: next = (prev == RIGHT) ? DOWN : UP; break;
+                case BOTTOM_RIGHT | TOP_RIGHT: next = DOWN; insert(out, point(next)); break;
+                case BOTTOM_RIGHT | TOP_RIGHT | TOP_LEFT: next = DOWN; insert(out, point(next)); break;
+                case BOTTOM_RIGHT | BOTTOM_LEFT: next = LEFT; insert(out, point(next)); break;
+                case BOTTOM_RIGHT | BOTTOM_LEFT | TOP_LEFT: next = UP; insert(out, point(next)); break;
+                case BOTTOM_RIGHT | BOTTOM_LEFT | TOP_RIGHT: next = LEFT; insert(out, point(next)); break;
+                default: break;
+            }
+            switch (next) {
+                case UP: top--; bottom--; break;
+                case DOWN: top++; bottom++; break;
+                case LEFT: left--; right--; break;
+                case RIGHT: left++; right++; break;
+                default: break;
+            }
+            prev = next;
+            if (++it > cap) return -1;
+        } while (left != sx - 1 || bottom != sy);
+        if (!out.empty()) {
+            if (out[0].x != out.back().x || out[0].y != out.back().y) out.push_back(out[0]);
+        }
+        return 1;
+    }
+};
+
+/* LSbox::isMotionRegular (box.cpp:1155-1190) */
+bool is_motion_regular(Ctx& C, Grain& G, int old_contour, int new_contour, double old_volume, double new_volume) {
+    double mullins = (new_volume - old_volume) / C.dt * (3 / kPi);
+    double contourRatio = ((double)new_contour) / old_contour;
+    double volumeRatio = new_volume / old_volume;
+    if (mullins > -6.0) return true;
+    if (volumeRatio < 0.10) return false;
+    if (contourRatio < 0.35) return false;
+    if (contourRatio > 0.7) return true;
+    G.motion_regular = false;
+    return true;
+}
+
+/* LSbox::clearContourGrainArea (box.cpp:1883-1904) */
+bool clear_contour_grain_area(Ctx& C, Grain& G) {
+    if (G.contour.size() == 0) return true;
+    double minx = G.contour[0].x, maxx = minx, miny = G.contour[0].y, maxy = miny;
+    for (const Pt& p : G.contour) {
+        minx = std::min(minx, p.x); maxx = std::max(maxx, p.x);
+        miny = std::min(miny, p.y); maxy = std::max(maxy, p.y);
+    }
+    DBuf<double>& in = *G.in;
+    for (int i = (int)(miny - 1); i < maxy + 1; i++)
+        for (int j = (int)(minx - 1); j < maxx + 1; j++) {
+            size_t k;
+            if (!in.flat(i, j, k)) return false;  // uncaught std::out_of_range in the reference
+            if (in.v[k] >= 0) in.v[k] = -C.h;
+        }
+    return true;
+}
+
+/* LSbox::extractContour (box.cpp:1089-1153) */
+void extract_contour(Ctx& C, Grain& G) {
+    G.motion_regular = true;
+    int old_len = (int)G.contour.size();
+    March ms(*G.in);
+    int r = ms.contour(G.contour);
+    if (r < 0) { G.error = 2; G.exists = false; return; }
+    G.exists = (r == 1);
+    if (!G.exists) return;
+    double newVolume = contour_volume(G.contour, C.h);
+    if (!is_motion_regular(C, G, old_len, (int)G.contour.size(), G.volume, newVolume)) {
+        do {
+            if (!clear_contour_grain_area(C, G)) { G.error = 3; G.exists = false; return; }
+            r = ms.contour(G.contour);
+            if (r < 0) { G.error = 2; G.exists = false; return; }
+            G.exists = (r == 1);
+            if (!G.exists) return;
+            newVolume = contour_volume(G.contour, C.h);
+        } while (!is_motion_regular(C, G, old_len, (int)G.contour.size(), G.volume, newVolume));
+    }
+    int m = C.m;
+    if (G.newXMin < 0) G.newXMin = 0;
+    if (G.newYMin < 0) G.newYMin = 0;
+    if (G.newXMax > m) G.newXMax = m;
+    if (G.newYMax > m) G.newYMax = m;
+    G.out->resize(G.newXMin, G.newYMin, G.newXMax, G.newYMax);
+    G.out->resizeToSquare(C.m);
+    double per = 0;  // ExplicitGrainBoundary::computePerimeter (grainBoundary.cpp:224-231)
+    for (size_t i = 0; i + 1 < G.contour.size(); ++i) per += seglen(G.contour[i], G.contour[i + 1]) * C.h;
+    G.perimeter = per;
+}
+
+/* ExplicitGrainBoundary::buildDirectNeighbours (grainBoundary.cpp:85-199), isotropic:
+ * GBmobilityModel and GBEnergyReadShockley both return 1.0 (box.cpp:1789-1792, 1826-1828). */
+void build_direct_neighbours(Ctx& C, Grain& G) {
+    G.neigh.clear();
+    DBuf<double>& D = *G.in;
+    size_t nc = G.contour.size();
+    for (size_t i = 0; i + 1 < nc; i++) {
+        double px = G.contour[i].x, py = G.contour[i].y;
+        int pxGrid = (int)px, pyGrid = (int)py;
+        int cand = -1;
+        size_t k;
+        bool ok = true;
+        if (!D.flat(pyGrid, pxGrid, k)) ok = false;
+        else if (D.v[k] > 0) {
+            pxGrid = (int)(px + 1);
+            if (!D.flat(pyGrid, pxGrid, k)) ok = false;
+            else if (D.v[k] > 0) {
+                pyGrid = (int)(py + 1);
+                if (!D.flat(pyGrid, pxGrid, k)) ok = false;
+                else if (D.v[k] > 0) pxGrid = (int)px;
+            }
+        }
+        if (ok) {  // LSbox::getNeighbourAt (box.cpp:2190-2195)
+            if (pyGrid < 0 || pyGrid >= G.idl.ymax || pxGrid < 0 || pxGrid >= G.idl.xmax) ok = false;
+            else if (!G.idl.flat(pyGrid, pxGrid, k)) ok = false;
+            else {
+                uint32_t nid = G.idl.v[k];
+                // grainhdl::getGrainByID (grainhdl.h:153-160)
+                if (nid == 0) cand = 0;
+                else if (nid < C.g.size() && C.g[nid].present) cand = (int)nid;
+                else ok = false;
+            }
+        }
+        if (!ok) {  // catch -> markAsInvalidMotion (box.cpp:1939-1945)
+            G.motion_regular = false;
+            return;
+        }
+        G.contour[i].mob = 1.0;
+        G.contour[i].energy = 1.0;
+        double line_length = seglen(G.contour[i], G.contour[i + 1]) * C.h;
+        size_t e = 0;
+        for (; e < G.neigh.size(); ++e)
+            if (G.neigh[e].id == cand) break;
+        if (e == G.neigh.size()) G.neigh.push_back(Neighbour{cand, 0.0});
+        G.neigh[e].length += line_length;
+    }
+    G.contour[nc - 1].energy = G.contour[0].energy;
+    G.contour[nc - 1].mob = G.contour[0].mob;
+}
+
+/* LSbox::computeVolumeAndEnergy (box.cpp:1201-1228) */
+void compute_volume_and_energy(Ctx& C, Grain& G) {
+    if (!G.exists || !G.motion_regular) return;
+    G.energy = 0;
+    double newVolume = std::fabs(contour_volume(G.contour, C.h));
+    build_direct_neighbours(C, G);
+    double e = 0;  // ExplicitGrainBoundary::computeEnergy (grainBoundary.cpp:215-222)
+    for (size_t i = 0; i + 1 < G.contour.size(); ++i) e += seglen(G.contour[i], G.contour[i + 1]) * G.contour[i].energy * C.h;
+    G.energy = e;
+    double dA = newVolume - G.volume;
+    dA = dA / C.dt;
+    dA = dA * (3 / kPi);
+    G.volume = newVolume;
+    unsigned int lp = (unsigned int)C.loop;
+    unsigned long at = C.p.analysis_timestep > 0 ? (unsigned long)C.p.analysis_timestep : 1ul;
+    if ((unsigned long)(lp - 1u) % at == 0 || lp == 0) G.meanDa = 0;
+    G.meanDa += dA;
+}
+
+/* LSbox::executeRedistancing (box.cpp:1259-1399) */
+void execute_redistancing(Ctx& C, Grain& G) {
+    if (!G.exists) return;
+    double h = C.h;
+    DBuf<double>& in = *G.in;
+    DBuf<double>& o = *G.out;
+    o.fill(-1.0);
+    int ixmin = in.xmin < o.xmin ? o.xmin : in.xmin;
+    int iymin = in.ymin < o.ymin ? o.ymin : in.ymin;
+    int ixmax = in.xmax < o.xmax ? in.xmax : o.xmax;
+    int iymax = in.ymax < o.ymax ? in.ymax : o.ymax;
+    double cand, slope, d0;
+    // y-direction forward (1291-1319)
+    for (int j = ixmin; j < o.xmax; j++) {
+        for (int i = iymin; i < o.ymax - 1; i++) {
+            if (j < ixmax && i < iymax - 1) {
+                if (in.d(i, j) * in.d(i + 1, j) <= 0.0) {
+                    slope = (in.d(i + 1, j) - in.d(i, j)) / h;
+                    d0 = -in.d(i, j) / slope;
+                    if (std::fabs(o.d(i, j)) > std::fabs(d0)) o.d(i, j) = -d0 * sgn(slope);
+                }
+                cand = o.d(i, j) + (sgn(in.d(i + 1, j)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i + 1, j))) o.d(i + 1, j) = cand;
+            } else {
+                cand = o.d(i, j) + (sgn(o.d(i + 1, j)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i + 1, j))) o.d(i + 1, j) = cand;
+            }
+        }
+    }
+    // y-direction backward (1321-1338)
+    for (int j = ixmin; j < o.xmax; j++) {
+        for (int i = iymax - 1; i > o.ymin; i--) {
+            if (j < ixmax && i > iymin) {
+                cand = o.d(i, j) + (sgn(in.d(i - 1, j)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i - 1, j))) o.d(i - 1, j) = cand;
+            } else {
+                cand = o.d(i, j) + (sgn(o.d(i - 1, j)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i - 1, j))) o.d(i - 1, j) = cand;
+            }
+        }
+    }
+    // x-direction forward (1339-1367)
+    for (int i = iymin; i < o.ymax; i++) {
+        for (int j = ixmin; j < o.xmax - 1; j++) {
+            if (j < ixmax - 1 && i < iymax) {
+                if (in.d(i, j) * in.d(i, j + 1) <= 0.0) {
+                    slope = (in.d(i, j + 1) - in.d(i, j)) / h;
+                    d0 = -in.d(i, j) / slope;
+                    if (std::fabs(o.d(i, j)) > std::fabs(d0)) o.d(i, j) = -d0 * sgn(slope);
+                }
+                cand = o.d(i, j) + (sgn(in.d(i, j + 1)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i, j + 1))) o.d(i, j + 1) = cand;
+            } else {
+                cand = o.d(i, j) + (sgn(o.d(i, j + 1)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i, j + 1))) o.d(i, j + 1) = cand;
+            }
+        }
+    }
+    // x-direction backward (1369-1388)
+    for (int i = iymin; i < o.ymax; i++) {
+        for (int j = ixmax - 1; j > o.xmin; j--) {
+            if (j > ixmin && i < iymax) {
+                cand = o.d(i, j) + (sgn(in.d(i, j - 1)) * h);
+                if (std::fabs(cand) < std::fabs(o.d(i, j - 1))) o.d(i, j - 1) = cand;
+            } else {
+                cand = o.d(i, j) + sgn(o.d(i, j - 1)) * h;
+                if (std::fabs(cand) < std::fabs(o.d(i, j - 1))) o.d(i, j - 1) = cand;
+            }
+        }
+    }
+    // clampValues (dimensionalBufferReal.h:43-56)
+    for (double& v : o.v) {
+        if (v < -C.delta) { v = -C.delta; continue; }
+        if (v > C.delta) { v = C.delta; continue; }
+    }
+    in.resize(o.xmin, o.ymin, o.xmax, o.ymax);  // box.cpp:1393-1395
+}
+
+/* LSbox::executeConvolution, isotropic branch (box.cpp:408-536) */
+void execute_convolution(Ctx& C, Grain& G) {
+    if (!G.exists) return;
+    int n = G.out->xmax - G.out->xmin;
+    convolve_box(n, C.p.convolution_mode, C.dt, C.R, G.in->v.data(), G.out->v.data());
+    G.idl.resize(G.out->xmin, G.out->ymin, G.out->xmax, G.out->ymax);  // reizeIDLocalToDistanceBuffer
+}
+
+template <class F>
+void for_grains(Ctx& C, F f) {
+    int N = (int)C.g.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int id = 1; id < N; ++id) f(C.g[id]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ggxo_create(const ggxo_params* p, int max_id, void** ctx) {
+    if (!p || !ctx || max_id < 1) return -1;
+    Ctx* C = new Ctx();
+    C->p = *p;
+    C->max_id = max_id;
+    C->R = p->real_domain_size;
+    C->gb = p->grid_blowup;
+    C->m = C->R + 2 * C->gb;         // ngridpoints (grainhdl.cpp:136)
+    C->h = 1.0 / double(C->R);       // grainhdl.cpp:127
+    C->delta = p->grid_blowup * 1 / double(C->R);  // grainhdl.cpp:133
+    C->dt = p->dt;
+    C->g.resize((size_t)max_id + 1);
+    for (int i = 0; i <= max_id; ++i) C->g[i].id = i;
+    C->g[0].present = true;  // the boundary grain (grainhdl.cpp:137)
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    *ctx = C;
+    return 0;
+}
+
+void ggxo_destroy(void* ctx) { delete (Ctx*)ctx; }
+
+int ggxo_add_grain(void* ctx, int id, int nverts, const double* xy, const double* q) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id || nverts < 2) return -1;
+    std::vector<Pt> v((size_t)nverts);
+    for (int k = 0; k < nverts; ++k) {
+        double x = xy[2 * k], y = xy[2 * k + 1];
+        // grainhdl::readMicrostructure clamps to [0,1] (grainhdl.cpp:645-652)
+        if (x < 0) x = 0;
+        if (y < 0) y = 0;
+        if (x > 1) x = 1;
+        if (y > 1) y = 1;
+        v[k] = Pt(x, y, 0, 0);
+    }
+    Grain& G = C.g[id];
+    G.present = true;
+    G.exists = true;
+    for (int k = 0; k < 4; ++k) G.q[k] = q ? q[k] : (k == 0 ? 1.0 : 0.0);
+    construct_grain(C, G, v);
+    return 0;
+}
+
+int ggxo_init(void* ctx) {
+    Ctx& C = *(Ctx*)ctx;
+    for_grains(C, [&](Grain& G) { if (G.present) distance_function(C, G); });
+    // find_neighbors (grainhdl.cpp:1258-1283): every grain whose OUTPUT box overlaps (closed
+    // intervals, RTree.h Overlap) seeds m_secondOrderNeighbours; order = ascending ID.
+    int N = (int)C.g.size();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int a = 1; a < N; ++a) {
+        Grain& A = C.g[a];
+        if (!A.present) continue;
+        A.second.clear();
+        for (int b = 1; b < N; ++b) {
+            if (b == a || !C.g[b].present) continue;
+            const DBuf<double>& oa = *A.out;
+            const DBuf<double>& ob = *C.g[b].out;
+            if (oa.xmin > ob.xmax || ob.xmin > oa.xmax || oa.ymin > ob.ymax || ob.ymin > oa.ymax) continue;
+            A.second.push_back(b);
+        }
+    }
+    return 0;
+}
+
+int ggxo_step(void* ctx, int loop) {
+    Ctx& C = *(Ctx*)ctx;
+    C.loop = loop;
+    auto swap = [&](Grain& G) { std::swap(G.in, G.out); };
+    // gridCoarsement else-branch -> switchDistancebuffer (grainhdl.cpp:1478-1480)
+    for_grains(C, [&](Grain& G) { if (G.present && G.exists) swap(G); });
+    // convolution (grainhdl.cpp:755-781)
+    for_grains(C, [&](Grain& G) { if (G.present) execute_convolution(C, G); });
+    for_grains(C, [&](Grain& G) { if (G.present && G.exists) swap(G); });
+    // updateSecondOrderNeighbors (grainhdl.cpp:1243-1256)
+    for_grains(C, [&](Grain& G) { if (G.present) second_order_neighbours(C, G); });
+    // comparison_box (grainhdl.cpp:815-828); executeSetComparison of a vanished grain only
+    // touches that grain's buffers, which are deleted before anyone reads them.
+    for_grains(C, [&](Grain& G) {
+        if (G.present && G.exists) { execute_comparison(C, G); execute_set_comparison(C, G); }
+    });
+    for_grains(C, [&](Grain& G) { if (G.present && G.exists) swap(G); });
+    // level_set (grainhdl.cpp:830-848): delete grains that vanished in the previous step
+    for_grains(C, [&](Grain& G) {
+        if (!G.present) return;
+        if (!G.exists) {
+            G.present = false;
+            G.bufA = DBuf<double>(); G.bufB = DBuf<double>(); G.idl = DBuf<uint32_t>();
+            G.in = &G.bufA; G.out = &G.bufB;
+        } else {
+            extract_contour(C, G);
+        }
+    });
+    // redistancing (grainhdl.cpp:850-869)
+    for_grains(C, [&](Grain& G) {
+        if (!G.present) return;
+        compute_volume_and_energy(C, G);
+        execute_redistancing(C, G);
+    });
+    int err = 0;
+    for (auto& G : C.g) err |= G.error;
+    return err ? -2 : 0;
+}
+
+int ggxo_get_info(void* ctx, int id, ggxo_grain_info* o) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    std::memset(o, 0, sizeof(*o));
+    o->present = G.present;
+    o->exists = G.exists;
+    o->motion_regular = G.motion_regular;
+    o->intersects_boundary = G.intersects_boundary;
+    o->xmin = G.out->xmin; o->xmax = G.out->xmax; o->ymin = G.out->ymin; o->ymax = G.out->ymax;
+    o->id_xmin = G.idl.xmin; o->id_xmax = G.idl.xmax; o->id_ymin = G.idl.ymin; o->id_ymax = G.idl.ymax;
+    o->contour_len = (int)G.contour.size();
+    o->n_neighbours = (int)G.neigh.size();
+    o->n_compare = (int)G.cmp.size();
+    o->error = G.error;
+    o->volume = G.volume; o->perimeter = G.perimeter; o->energy = G.energy; o->mean_da = G.meanDa;
+    return 0;
+}
+
+int ggxo_grain_phi(void* ctx, int id, double* out) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    std::memcpy(out, G.out->v.data(), G.out->v.size() * sizeof(double));
+    return (int)G.out->v.size();
+}
+
+int ggxo_grain_idlocal(void* ctx, int id, uint32_t* out) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    std::memcpy(out, G.idl.v.data(), G.idl.v.size() * sizeof(uint32_t));
+    return (int)G.idl.v.size();
+}
+
+int ggxo_grain_contour(void* ctx, int id, double* xy) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    for (size_t k = 0; k < G.contour.size(); ++k) { xy[2 * k] = G.contour[k].x; xy[2 * k + 1] = G.contour[k].y; }
+    return (int)G.contour.size();
+}
+
+int ggxo_grain_neighbours(void* ctx, int id, int32_t* ids, double* lengths) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    for (size_t k = 0; k < G.neigh.size(); ++k) { ids[k] = G.neigh[k].id; lengths[k] = G.neigh[k].length; }
+    return (int)G.neigh.size();
+}
+
+int ggxo_grain_compare_list(void* ctx, int id, int32_t* ids) {
+    Ctx& C = *(Ctx*)ctx;
+    if (id < 1 || id > C.max_id) return -1;
+    Grain& G = C.g[id];
+    for (size_t k = 0; k < G.cmp.size(); ++k) ids[k] = G.cmp[k];
+    return (int)G.cmp.size();
+}
+
+int ggxo_current_grains(void* ctx) {
+    Ctx& C = *(Ctx*)ctx;
+    int c = 0;
+    for (size_t i = 1; i < C.g.size(); ++i) c += C.g[i].present ? 1 : 0;
+    return c;
+}
+
+int ggxo_convolve(int n, int mode, double dt, int R, const double* in, double* out) {
+    if (n < 1) return -1;
+    convolve_box(n, mode, dt, R, in, out);
+    return 0;
+}
+
+int ggxo_dft(int n, int sign, const double* in_ri, double* out_ri) {
+    if (n < 1) return -1;
+    FftPlan pl(n);
+    std::vector<Cplx> a((size_t)n), work, scratch;
+    for (int k = 0; k < n; ++k) { a[k].re = in_ri[2 * k]; a[k].im = in_ri[2 * k + 1]; }
+    fft1d(a.data(), n, 1, sign < 0 ? -1 : 1, pl, work, scratch);
+    for (int k = 0; k < n; ++k) { out_ri[2 * k] = a[k].re; out_ri[2 * k + 1] = a[k].im; }
+    return 0;
+}
+
+}  // extern "C"

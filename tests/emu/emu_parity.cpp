@@ -1,0 +1,131 @@
+// Runs the libggx kernels on the CPU (through the fiber stand-in for the HIP runtime) next to
+// the oracle and compares every grain after every step.  Test infrastructure.
+//   emu_parity <ngrains> <steps> <seed> [gb] [ppg] [lognormal_sigma] [fp32]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ggx.h"
+#include "ggx_oracle.h"
+
+int main(int argc, char** argv) {
+    int N = argc > 1 ? atoi(argv[1]) : 30;
+    int steps = argc > 2 ? atoi(argv[2]) : 3;
+    unsigned long long seed = argc > 3 ? strtoull(argv[3], 0, 10) : 1001;
+    int gb = argc > 4 ? atoi(argv[4]) : 7;
+    int ppg = argc > 5 ? atoi(argv[5]) : 20;
+    double sigma = argc > 6 ? atof(argv[6]) : 0.0;
+    int fp32 = argc > 7 ? atoi(argv[7]) : 0;
+    ggx_network* net = nullptr;
+    if (ggx_network_generate(N, seed, sigma, &net)) { printf("generate failed\n"); return 2; }
+    int ng = ggx_network_count(net);
+    ggx_settings s;
+    ggx_settings_default(&s);
+    s.number_of_particles = ng;
+    s.number_of_points_per_grain = ppg;
+    s.number_of_timesteps = steps;
+    s.analysis_timestep = 1;
+    s.domain_border_size = gb;
+    s.microstructure_gen_mode = 0;
+    s.convolution_mode = 2;
+    s.is_isotropic_network = 1;
+    s.breakup_number = 0;
+    ggx_derived d;
+    ggx_settings_derive(&s, &d);
+    ggx_options o;
+    ggx_options_default(&o);
+    o.precision = fp32 ? GGX_PREC_FP32 : GGX_PREC_FP64;
+    ggx_ctx* ctx = nullptr;
+    int rc = ggx_create(&s, &o, &ctx);
+    if (rc) { printf("create failed %d\n", rc); return 2; }
+    rc = ggx_load_network(ctx, net);
+    if (rc) { printf("load failed %d: %s\n", rc, ggx_last_error(ctx)); return 2; }
+    ggxo_params p;
+    memset(&p, 0, sizeof p);
+    p.real_domain_size = d.real_domain_size;
+    p.grid_blowup = gb;
+    p.convolution_mode = 2;
+    p.dt = d.dt;
+    p.analysis_timestep = 1;
+    p.isotropic = 1;
+    p.nthreads = 1;
+    void* oc = nullptr;
+    ggxo_create(&p, ng, &oc);
+    for (int i = 0; i < ng; ++i) {
+        int id, nv;
+        const double *xy, *q;
+        ggx_network_grain(net, i, &id, &nv, &xy, &q);
+        ggxo_add_grain(oc, id, nv, xy, q);
+    }
+    ggxo_init(oc);
+    printf("grains %d R %d m %d dt %.6g\n", ng, d.real_domain_size, d.ngridpoints, d.dt);
+    int bad_total = 0;
+    std::vector<double> a(1 << 20), b(1 << 20);
+    std::vector<uint32_t> ia(1 << 20), ib(1 << 20);
+    std::vector<int> na(4096), nb(4096);
+    std::vector<double> la(4096), lb(4096);
+    for (int t = 0; t < steps; ++t) {
+        rc = ggx_step(ctx, 1);
+        if (rc) { printf("step %d failed %d: %s\n", t, rc, ggx_last_error(ctx)); return 3; }
+        int orc = ggxo_step(oc, t);
+        double maxerr = 0, maxvol = 0;
+        int bad = 0, idmis = 0, cmpmis = 0;
+        for (int id = 1; id <= ng; ++id) {
+            ggx_grain_info gi;
+            ggxo_grain_info oi;
+            ggx_get_info(ctx, id, &gi);
+            ggxo_get_info(oc, id, &oi);
+            if (gi.present != oi.present || (oi.present && gi.exists != oi.exists)) {
+                if (bad < 10) printf("  t%d g%d present/exists %d/%d vs oracle %d/%d\n", t, id, gi.present, gi.exists, oi.present, oi.exists);
+                bad++; continue;
+            }
+            if (!oi.present || !oi.exists) continue;
+            if (gi.xmin != oi.xmin || gi.xmax != oi.xmax || gi.ymin != oi.ymin || gi.ymax != oi.ymax) {
+                if (bad < 10) printf("  t%d g%d box [%d %d %d %d] vs [%d %d %d %d]\n", t, id, gi.xmin, gi.xmax, gi.ymin, gi.ymax, oi.xmin, oi.xmax, oi.ymin, oi.ymax);
+                bad++; continue;
+            }
+            if (gi.contour_len != oi.contour_len || gi.n_neighbours != oi.n_neighbours || gi.motion_regular != oi.motion_regular ||
+                gi.intersects_boundary != oi.intersects_boundary || gi.n_compare != oi.n_compare) {
+                if (bad < 10) printf("  t%d g%d clen %d/%d nnb %d/%d reg %d/%d bnd %d/%d ncmp %d/%d\n", t, id, gi.contour_len, oi.contour_len,
+                                     gi.n_neighbours, oi.n_neighbours, gi.motion_regular, oi.motion_regular, gi.intersects_boundary,
+                                     oi.intersects_boundary, gi.n_compare, oi.n_compare);
+                bad++;
+            }
+            double dv = fabs(gi.volume - oi.volume) / (fabs(oi.volume) + 1e-300);
+            if (dv > maxvol) maxvol = dv;
+            if (!fp32 && (fabs(gi.perimeter - oi.perimeter) > 1e-9 || fabs(gi.energy - oi.energy) > 1e-9 || dv > 1e-9)) {
+                if (bad < 10) printf("  t%d g%d vol %.15g/%.15g per %.15g/%.15g en %.15g/%.15g\n", t, id, gi.volume, oi.volume, gi.perimeter, oi.perimeter, gi.energy, oi.energy);
+                bad++;
+            }
+            int cnt = ggx_get_phi(ctx, id, a.data(), (int)a.size());
+            int cnt2 = ggxo_grain_phi(oc, id, b.data());
+            if (cnt != cnt2) { bad++; continue; }
+            for (int k = 0; k < cnt; ++k) maxerr = fmax(maxerr, fabs(a[k] - b[k]));
+            int c1 = ggx_get_idlocal(ctx, id, ia.data(), (int)ia.size());
+            int c2 = ggxo_grain_idlocal(oc, id, ib.data());
+            if (c1 != c2) { if (bad < 10) printf("  t%d g%d idlocal size %d vs %d\n", t, id, c1, c2); bad++; }
+            else for (int k = 0; k < c1; ++k) idmis += ia[k] != ib[k];
+            int k1 = ggx_get_neighbours(ctx, id, na.data(), la.data(), 4096);
+            int k2 = ggxo_grain_neighbours(oc, id, nb.data(), lb.data());
+            if (k1 != k2) bad++;
+            else for (int k = 0; k < k1; ++k) {
+                if (na[k] != nb[k] || (!fp32 && fabs(la[k] - lb[k]) > 1e-9)) { if (bad < 10) printf("  t%d g%d neighbour %d: %d %.12g vs %d %.12g\n", t, id, k, na[k], la[k], nb[k], lb[k]); bad++; break; }
+            }
+            k1 = ggx_get_compare_list(ctx, id, na.data(), 4096);
+            k2 = ggxo_grain_compare_list(oc, id, nb.data());
+            if (k1 != k2) cmpmis++;
+            else for (int k = 0; k < k1; ++k) if (na[k] != nb[k]) { cmpmis++; break; }
+        }
+        printf("step %d: grains %d/%d  max|dphi| %.3e  max rel dvol %.3e  idlocal mismatches %d  cmp-list mismatches %d  bad %d  (oracle rc %d)\n",
+               t, ggx_current_grains(ctx), ggxo_current_grains(oc), maxerr, maxvol, idmis, cmpmis, bad, orc);
+        if (!fp32 && maxerr > 1e-10) bad++;
+        bad_total += bad + cmpmis;
+    }
+    ggx_destroy(ctx);
+    ggxo_destroy(oc);
+    ggx_network_free(net);
+    printf(bad_total ? "PARITY FAILED\n" : "PARITY OK\n");
+    return bad_total ? 1 : 0;
+}

@@ -1,0 +1,153 @@
+// CPU execution of the libggx HIP sources, for tests only.
+//
+// This header stands in for <hip/hip_runtime.h> when tests/emu/Makefile compiles the product
+// sources with g++.  Each workgroup runs its lanes as cooperative fibers (ucontext): a lane
+// runs until __syncthreads() or its end, then the next lane runs, so barrier semantics and
+// LDS sharing are those of the device.  Workgroups run one after another; LDS is poisoned at
+// every launch.  Nothing here is linked into libggx.so.
+#pragma once
+#include <ucontext.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct double2 { double x, y; };
+struct int4 { int x, y, z, w; };
+inline double2 make_double2(double a, double b) { return double2{a, b}; }
+inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
+
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline long long min(long long a, long long b) { return a < b ? a : b; }
+inline long long max(long long a, long long b) { return a > b ? a : b; }
+
+namespace emu {
+struct Fiber {
+    ucontext_t ctx;
+    bool done;
+};
+inline ucontext_t g_main;
+inline std::vector<Fiber> g_fibers;
+inline std::vector<char*> g_stacks;
+inline int g_cur = 0;
+inline dim3 g_tid, g_bid, g_bdim, g_gdim;
+inline std::function<void()>* g_body = nullptr;
+inline std::vector<unsigned char> g_dyn;
+constexpr size_t kStack = 256 * 1024;
+
+inline void entry() {
+    (*g_body)();
+    g_fibers[g_cur].done = true;
+    swapcontext(&g_fibers[g_cur].ctx, &g_main);
+}
+inline void sync() { swapcontext(&g_fibers[g_cur].ctx, &g_main); }
+
+inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> body) {
+    unsigned nt = block.x * block.y * block.z;
+    while (g_stacks.size() < nt) g_stacks.push_back((char*)std::malloc(kStack));
+    g_fibers.assign(nt, Fiber{});
+    g_body = &body;
+    g_gdim = grid;
+    g_bdim = block;
+    for (unsigned b = 0; b < grid.x; ++b) {
+        g_bid = dim3(b, 0, 0);
+        g_dyn.assign(shmem + 16, 0xCD);
+        for (unsigned t = 0; t < nt; ++t) {
+            getcontext(&g_fibers[t].ctx);
+            g_fibers[t].ctx.uc_stack.ss_sp = g_stacks[t];
+            g_fibers[t].ctx.uc_stack.ss_size = kStack;
+            g_fibers[t].ctx.uc_link = &g_main;
+            g_fibers[t].done = false;
+            makecontext(&g_fibers[t].ctx, (void (*)())entry, 0);
+        }
+        bool any = true;
+        while (any) {
+            any = false;
+            for (unsigned t = 0; t < nt; ++t) {
+                if (g_fibers[t].done) continue;
+                g_cur = (int)t;
+                g_tid = dim3(t, 0, 0);
+                swapcontext(&g_main, &g_fibers[t].ctx);
+                if (!g_fibers[t].done) any = true;
+            }
+        }
+    }
+    g_body = nullptr;
+}
+}  // namespace emu
+
+#define threadIdx (emu::g_tid)
+#define blockIdx (emu::g_bid)
+#define blockDim (emu::g_bdim)
+#define gridDim (emu::g_gdim)
+#define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)emu::g_dyn.data();
+inline void __syncthreads() { emu::sync(); }
+inline void __threadfence() {}
+inline void __threadfence_block() {}
+inline int __ffs(unsigned int v) { return __builtin_ffs((int)v); }
+inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { auto o = *p; *p += v; return o; }
+inline int atomicAdd(int* p, int v) { int o = *p; *p += v; return o; }
+inline int atomicMin(int* p, int v) { int o = *p; if (v < o) *p = v; return o; }
+inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
+inline int atomicOr(int* p, int v) { int o = *p; *p |= v; return o; }
+inline unsigned atomicAnd(unsigned* p, unsigned v) { unsigned o = *p; *p &= v; return o; }
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+    emu::launch(grid, block, shmem, [&]() { kernel(__VA_ARGS__); })
+
+typedef int hipError_t;
+typedef int hipStream_t_;
+typedef hipStream_t_* hipStream_t;
+typedef double* hipEvent_t;
+constexpr hipError_t hipSuccess = 0;
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize };
+struct hipDeviceProp_t {
+    int multiProcessorCount;
+    size_t sharedMemPerBlock;
+    size_t maxSharedMemoryPerMultiProcessor;
+};
+inline const char* hipGetErrorString(hipError_t) { return "emu error"; }
+inline hipError_t hipMalloc(void** p, size_t n) { *p = std::calloc(n ? n : 1, 1); return *p ? 0 : 2; }
+inline hipError_t hipFree(void* p) { std::free(p); return 0; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return 0; }
+inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
+inline hipError_t hipSetDevice(int) { return 0; }
+inline hipError_t hipStreamCreate(hipStream_t* s) { *s = new int(0); return 0; }
+inline hipError_t hipStreamDestroy(hipStream_t s) { delete s; return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipDeviceSynchronize() { return 0; }
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new double(0); return 0; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+    *e = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    return 0;
+}
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return 0; }
+inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
+    p->multiProcessorCount = 4;
+    p->sharedMemPerBlock = 65536;
+    p->maxSharedMemoryPerMultiProcessor = 163840;
+    return 0;
+}
+inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
+inline hipError_t hipGetLastError() { return 0; }

@@ -1,0 +1,40 @@
+"""The C-ABI library loads and exports every symbol include/ggx.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from gragles2d_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ggx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ggx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_native.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_c_layout(lib):
+    # ggx_settings: 27 int32 + 11 doubles + 2 * 512 chars; natural alignment
+    assert ctypes.sizeof(_native.Settings) == 27 * 4 + 4 + 11 * 8 + 1024
+    assert ctypes.sizeof(_native.GrainInfo) == 16 * 4 + 4 * 8
+    assert ctypes.sizeof(_native.Options) == 6 * 4 + 8
+
+
+def test_errors_without_gpu_are_status_codes(lib):
+    s = _native.Settings()
+    lib.ggx_settings_default(ctypes.byref(s))
+    d = _native.Derived()
+    # NumberOfParticles == 0 -> GGX_E_CONFIG, no exception, no GPU call
+    assert lib.ggx_settings_derive(ctypes.byref(s), ctypes.byref(d)) == -1
+    assert lib.ggx_settings_read_xml(b"/nonexistent/parameters.xml", ctypes.byref(s)) == -7

@@ -1,0 +1,100 @@
+"""Host-side pieces of the boundary: parameters.xml, derived constants, mode-0 network files."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gragles2d_amd import grainhdl as G
+from gragles2d_amd.configs import WORKLOADS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_reads_reference_style_parameters(tmp_path):
+    # the reference's own sample file layout (params/parameters.xml keys, comments, spaces)
+    xml = """<?xml version="1.0" encoding="utf-8"?>
+<Parameters>
+   <StartTime> 0 </StartTime>
+   <NumberOfParticles> 101 </NumberOfParticles>
+   <NumberOfPointsPerGrain> 20 </NumberOfPointsPerGrain>
+   <AnalysisTimestep>1</AnalysisTimestep>
+   <!-- a comment with <tags> inside -->
+   <NumberOfTimesteps> 10000  </NumberOfTimesteps>
+   <DomainBorderSize> 7 </DomainBorderSize>
+   <MicrostructureGenMode>1</MicrostructureGenMode>
+   <ReadFromFilename>SyntheticTricrystal.bin</ReadFromFilename>
+   <HAGB_Energy>0.324</HAGB_Energy>
+   <ResearchMode> 2 </ResearchMode>
+   <ConvolutionMode> 7 </ConvolutionMode>
+   <MaxMisOrientation> 25 </MaxMisOrientation>
+</Parameters>
+"""
+    p = tmp_path / "parameters.xml"
+    p.write_text(xml)
+    s = G.read_settings(str(p))
+    assert s.number_of_particles == 101 and s.number_of_points_per_grain == 20
+    assert s.number_of_timesteps == 10000 and s.domain_border_size == 7
+    assert s.user_def_number_of_points_per_grain == 20          # Settings.cpp:122
+    assert s.research_mode == 1                                 # static bool (Settings.h:98)
+    assert s.convolution_mode == 3                              # clamped to E_INVALID_VALUE
+    assert s.hagb_energy == pytest.approx(0.324)
+    assert s.hagb_mobility == 1.0 and s.gaussian_kernel_time_slope == 0.8359  # defaults
+    assert s.read_from_filename == b"SyntheticTricrystal.bin"
+
+
+def test_settings_round_trip(tmp_path):
+    s = G.default_settings(**WORKLOADS["iso-2k"].settings_kw())
+    p = str(tmp_path / "PARAMETERS_SIM.xml")
+    from gragles2d_amd import _native as N
+    import ctypes
+    assert N.load().ggx_settings_write_xml(p.encode(), ctypes.byref(s), b"NETWORK.dat", 17, 1234) == 0
+    r = G.read_settings(p)
+    assert r.start_time == 17 and r.number_of_particles == 1234 and r.microstructure_gen_mode == 0
+    assert r.domain_border_size == s.domain_border_size and r.convolution_mode == 2
+    assert r.read_from_filename == b"NETWORK.dat"
+
+
+@pytest.mark.parametrize("name", list(WORKLOADS))
+def test_derived_constants(name):
+    w = WORKLOADS[name]
+    d = G.derive(G.default_settings(**w.settings_kw()))
+    R = int(math.sqrt(w.grains) * w.points_per_grain)           # grainhdl.cpp:77-78
+    assert d.real_domain_size == R
+    assert d.ngridpoints == R + 2 * w.border                     # grainhdl.cpp:136
+    assert d.h == 1.0 / R and d.delta == w.border / R
+    assert d.dt == 0.8 / (R * R) * w.points_per_grain / 2 / 5.0  # grainhdl.cpp:113-114
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.35])
+def test_generated_network_tiles_the_unit_square(sigma):
+    net = G.Network.generate(300, 42, sigma)
+    total = 0.0
+    for i in range(len(net)):
+        gid, pts, q = net.grain(i)
+        assert gid == i + 1
+        assert np.allclose(pts[0], pts[-1])                     # closed polygon
+        x, y = pts[:, 0], pts[:, 1]
+        total += 0.5 * abs(np.sum((y[:-1] + y[1:]) * (x[:-1] - x[1:])))
+        assert abs(np.linalg.norm(q) - 1) < 1e-12 and np.all(q >= 0)
+    assert abs(total - 1.0) < 1e-9
+
+
+def test_generation_is_deterministic():
+    a, b = G.Network.generate(50, 7), G.Network.generate(50, 7)
+    for i in range(len(a)):
+        assert np.array_equal(a.grain(i)[1], b.grain(i)[1])
+    c = G.Network.generate(50, 8)
+    assert not np.array_equal(a.grain(0)[1], c.grain(0)[1])
+
+
+def test_network_file_round_trip(tmp_path):
+    net = G.Network.generate(40, 3)
+    p = str(tmp_path / "NETWORK.dat")
+    net.write(p)
+    back = G.Network.read(p, len(net))
+    assert len(back) == len(net)
+    for i in range(len(net)):
+        ga, pa, qa = net.grain(i)
+        gb, pb, qb = back.grain(i)
+        assert ga == gb and np.array_equal(pa, pb) and np.array_equal(qa, qb)
