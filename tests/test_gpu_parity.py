@@ -49,10 +49,10 @@ def test_seeds_margins_and_lognormal(seed, gb, sigma):
 
 
 def test_iso2k_fp64_first_steps():
-    """Workload iso-2k (2000 grains, ~900x900, box sides 40-90) for its first 6 passes."""
+    """Workload iso-2k (2000 grains, ~900x900, box sides 40-90) for its first 12 passes."""
     w = WORKLOADS["iso-2k"]
     net = G.Network.generate(w.grains, w.seed, w.lognormal_sigma)
-    _run(net, 6, w.points_per_grain, w.border, oracle_threads=16)
+    _run(net, 12, w.points_per_grain, w.border, oracle_threads=16)
 
 
 def test_tiny_and_edge_networks():
