@@ -47,7 +47,7 @@ class Derived(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", c_int), ("batching", c_int), ("conv_engine", c_int), ("device", c_int),
-                ("lds_budget", c_int), ("use_graph", c_int), ("arena_headroom", c_dbl)]
+                ("lds_budget", c_int), ("use_graph", c_int), ("persistent", c_int), ("arena_headroom", c_dbl)]
 
 
 class GrainInfo(ctypes.Structure):

@@ -1,15 +1,16 @@
 // ggx.hip -- libggx: device context, arena management and the batched timestep.
 //
 // Replaces the bodies of the grainhdl phase loops (src/grainhdl.cpp:755-869, 1243-1256,
-// 1381-1394) with batched HIP launches.  Per step:
-//   k_prep      deletion of grains that vanished in the previous step, allocator reset
-//   k_lists     comparison lists (computeSecondOrderNeighbours)
-//   k_conv_*    convolution of every live box (executeConvolution)
-//   k_compare   comparison + set-comparison + new bounding box
-//   k_contour   marching squares, motion check, new box allocation, neighbours, volume
-//   k_redist    redistancing into the next arena
-// Grains are sorted by box size and launched in size buckets so that each bucket gets the
-// LDS its largest box needs.
+// 1381-1394) with batched HIP launches.  Per step, with no host round trip:
+//   k_reset / k_prep   deletion of grains that vanished in the previous step, survivor count,
+//                      filing of live grains into box-size buckets, allocator reset
+//   k_lists            comparison lists (computeSecondOrderNeighbours)
+//   k_conv_circulant   convolution of every live box (executeConvolution), one persistent
+//                      launch per bucket
+//   k_level            comparison, contour, motion check, new box, neighbours, volume and
+//                      redistancing (executeComparison .. executeRedistancing), fused, one
+//                      persistent launch per bucket
+// Each bucket's launches get the LDS its largest box needs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,13 +24,15 @@
 #include "ggx.h"
 #include "ggx_internal.h"
 #include "kernels.h"
+#include "level.h"
 
 using namespace ggx;
 
 namespace {
 
-const int kBucketEdges[] = {24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 1 << 30};
+const int kBucketEdges[] = {24, 32, 40, 48, 56, 64, 72, 88, 112, 144, 192, 256, 512};
 constexpr int kNumBuckets = sizeof(kBucketEdges) / sizeof(kBucketEdges[0]);
+static_assert(kNumBuckets <= kMaxBuckets, "bucket table");
 constexpr int kMaxKernN = 2048;  // largest box side with a precomputed circulant
 
 template <class T>
@@ -77,9 +80,17 @@ struct ggx_ctx {
     double *volume = nullptr, *perimeter = nullptr, *energy = nullptr, *meanda = nullptr;
     int *nb_cnt = nullptr, *nb_id = nullptr;
     double* nb_len = nullptr;
-    unsigned long long* tops = nullptr;
+    Sched* sched = nullptr;        // device counters of the current step
+    Sched* h_sched = nullptr;      // pinned copy, filled at the end of every ggx_step call
+    int* blists = nullptr;         // per-bucket work lists (kNumBuckets x nslots)
+    int* alist = nullptr;          // all live grains
     int* live = nullptr;
-    int* nprev_tmp = nullptr;
+    int *pg_list = nullptr, *pg_one = nullptr, *pg_cursor = nullptr;  // per-grain baseline lists
+    int known_max_n = 0;
+    long long updates_total = 0;
+    int lds_per_cu = 163840;
+    int static_conv[2] = {0, 0};   // static LDS bytes of k_conv_circulant<double/float>
+    int static_level[2] = {0, 0};  // static LDS bytes of k_level<double/float>
 
     // circulant kernels
     long long* kern_off = nullptr;
@@ -92,8 +103,9 @@ struct ggx_ctx {
     std::vector<int> h_live;
     std::vector<int> h_err;
 
-    // stats
-    hipEvent_t ev[8];
+    // stats: per-step phase events, read back at ggx_get_stats
+    std::vector<std::vector<hipEvent_t>> ev_pending;
+    std::vector<std::vector<hipEvent_t>> ev_free;
     ggx_stats st;
 };
 
@@ -126,11 +138,7 @@ void dfree(X*& p) {
     p = nullptr;
 }
 
-int contour_cap_of(int n) {
-    long long c = 16LL * n + 64;
-    long long c2 = 2LL * (n + 1) * (n + 1) + 2;
-    return (int)std::min(c, c2);
-}
+int contour_cap_of(int n) { return contour_cap(n); }
 
 long long aligned(long long v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -193,12 +201,13 @@ int raise_lds(ggx_ctx* ctx) {
     ctx->cu_count = prop.multiProcessorCount;
     int lim = (int)prop.sharedMemPerBlock;
     int smp = (int)prop.maxSharedMemoryPerMultiProcessor;
+    if (smp > 0) ctx->lds_per_cu = smp;
     if (smp > lim) lim = smp;
     if (ctx->opt.lds_budget > 0) lim = std::min(lim, ctx->opt.lds_budget);
     // allow the dynamic-LDS kernels to use up to `lim`
     const void* fns[] = {(const void*)k_conv_circulant<double>, (const void*)k_conv_circulant<float>,
-                         (const void*)k_contour<double>,        (const void*)k_contour<float>,
-                         (const void*)k_redist<double>,         (const void*)k_redist<float>};
+                         (const void*)k_level<double, true>, (const void*)k_level<float, true>,
+                         (const void*)k_level<double, false>, (const void*)k_level<float, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) {
@@ -207,131 +216,13 @@ int raise_lds(ggx_ctx* ctx) {
         }
     }
     ctx->lds_max = lim;
-    return GGX_OK;
-}
-
-struct Bucket {
-    int start, count, nmax;
-};
-
-std::vector<Bucket> make_buckets(const std::vector<int>& live, const std::vector<Box>& box) {
-    // live is sorted by n descending
-    std::vector<Bucket> out;
-    size_t i = 0;
-    while (i < live.size()) {
-        int n0 = box[live[i]].n;
-        int b = 0;
-        while (kBucketEdges[b] < n0) b++;
-        int lo = (b == 0) ? 0 : kBucketEdges[b - 1];
-        size_t j = i;
-        while (j < live.size() && box[live[j]].n > lo) j++;
-        out.push_back(Bucket{(int)i, (int)(j - i), n0});
-        i = j;
+    // static __shared__ bytes count against the same per-workgroup limit as the dynamic request
+    int* dst[] = {&ctx->static_conv[0], &ctx->static_conv[1], &ctx->static_level[0], &ctx->static_level[1]};
+    for (int k = 0; k < 4; ++k) {
+        hipFuncAttributes fa;
+        HIPCHK(hipFuncGetAttributes(&fa, fns[k]));
+        *dst[k] = (int)fa.sharedSizeBytes;
     }
-    return out;
-}
-
-template <class T>
-int launch_step(ggx_ctx* ctx) {
-    hipStream_t st = ctx->stream;
-    GridConst G = grid_const(ctx);
-    std::vector<int>& live = ctx->h_live;
-    int nlive = (int)live.size();
-    const std::vector<Bucket> buckets = make_buckets(live, ctx->h_box);
-    bool pergrain = ctx->opt.batching == GGX_BATCH_PERGRAIN;
-    T* P = (T*)ctx->P[ctx->cur];
-    T* Pn = (T*)ctx->P[ctx->cur ^ 1];
-    T* Tt = (T*)ctx->Tt;
-    uint32_t* Iold = ctx->I[ctx->icur];
-    uint32_t* Inew = ctx->I[ctx->icur ^ 1];
-    int lds_max = ctx->lds_max;
-
-    HIPCHK(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL(k_prep, dim3((ctx->nslots + 255) / 256), dim3(256), 0, st, ctx->nslots, ctx->flags, ctx->tops);
-    if (nlive > 0) {
-        if (pergrain) {
-            for (int i = 0; i < nlive; ++i)
-                hipLaunchKernelGGL(k_lists, dim3(1), dim3(64), 0, st, 1, ctx->live + i, ctx->box, ctx->flags,
-                                   ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
-        } else {
-            hipLaunchKernelGGL(k_lists, dim3((nlive + 63) / 64), dim3(64), 0, st, nlive, ctx->live, ctx->box,
-                               ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
-        }
-    }
-    HIPCHK(hipEventRecord(ctx->ev[1], st));
-    // convolution
-    for (const Bucket& bk : buckets) {
-        int n = bk.nmax;
-        long long kk = ((2LL * n + 4 + 1) & ~1LL);
-        long long need = (kk + 2LL * n * n) * (long long)sizeof(T);
-        long long lds = std::min<long long>(need, lds_max);
-        if ((kk + 4LL * n) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
-        int grid = pergrain ? 1 : bk.count;
-        int reps = pergrain ? bk.count : 1;
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_conv_circulant<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r,
-                               ctx->box, ctx->off, (const T*)P, Tt, ctx->kern_off, (const T*)ctx->kern, (int)lds);
-    }
-    HIPCHK(hipEventRecord(ctx->ev[2], st));
-    // comparison
-    for (const Bucket& bk : buckets) {
-        int grid = pergrain ? 1 : bk.count;
-        int reps = pergrain ? bk.count : 1;
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_compare<T>, dim3(grid), dim3(256), 0, st, ctx->live + bk.start + r, ctx->box,
-                               ctx->off, ctx->cmp_cnt, ctx->cmp, (const T*)Tt, P, Iold, Inew, ctx->off_prev, ctx->nprev,
-                               ctx->newbox, ctx->flags, G);
-    }
-    HIPCHK(hipEventRecord(ctx->ev[3], st));
-    // level set
-    ContourOut O;
-    O.pts = ctx->pts; O.coff = ctx->coff; O.ccap = ctx->ccap; O.clen = ctx->clen;
-    O.volume = ctx->volume; O.perimeter = ctx->perimeter; O.energy = ctx->energy; O.meanda = ctx->meanda;
-    O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
-    O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
-    O.tops = ctx->tops; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
-    for (const Bucket& bk : buckets) {
-        int n = bk.nmax;
-        long long nwords = ((long long)n * n + 31) / 32;
-        int cc = contour_cap_of(n);
-        long long lds = ((nwords * 4 + (long long)cc * 4 + 15) & ~15LL) + (long long)cc * 8 + 16;
-        if (lds > lds_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the contour kernel");
-        int grid = pergrain ? 1 : bk.count;
-        int reps = pergrain ? bk.count : 1;
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_contour<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r,
-                               ctx->box, ctx->off, P, Inew, ctx->newbox, O, G, ctx->loop, ctx->s.analysis_timestep);
-    }
-    HIPCHK(hipEventRecord(ctx->ev[4], st));
-    // redistancing
-    for (const Bucket& bk : buckets) {
-        int n = bk.nmax + 8;
-        long long lds = (long long)(n | 1) * n * (long long)sizeof(T);
-        if (lds > lds_max) lds = 0;
-        int grid = pergrain ? 1 : bk.count;
-        int reps = pergrain ? bk.count : 1;
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(k_redist<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->live + bk.start + r, ctx->box,
-                               ctx->off, (const T*)P, ctx->box_next, ctx->off_next, ctx->flags, Pn, G, (int)lds);
-    }
-    HIPCHK(hipEventRecord(ctx->ev[5], st));
-    HIPCHK(hipGetLastError());
-    return GGX_OK;
-}
-
-// Host-side copy of the per-grain state needed to schedule the next step.
-int pull_state(ggx_ctx* ctx) {
-    hipStream_t st = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->h_flags.data(), ctx->flags, ctx->nslots * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(ctx->h_box.data(), ctx->box, ctx->nslots * sizeof(Box), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(ctx->h_err.data(), ctx->errs, ctx->nslots * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (int g = 1; g < ctx->nslots; ++g)
-        if (ctx->h_err[g]) {
-            char buf[160];
-            std::snprintf(buf, sizeof buf, "grain %d: device error %d at loop %d", g, ctx->h_err[g], ctx->loop);
-            return fail(ctx, GGX_E_BOUNDS, buf);
-        }
     return GGX_OK;
 }
 
@@ -339,8 +230,205 @@ void build_live(ggx_ctx* ctx) {
     ctx->h_live.clear();
     for (int g = 1; g < ctx->nslots; ++g)
         if ((ctx->h_flags[g] & kPresent) && (ctx->h_flags[g] & kExists)) ctx->h_live.push_back(g);
-    const std::vector<Box>& b = ctx->h_box;
-    std::stable_sort(ctx->h_live.begin(), ctx->h_live.end(), [&](int a, int c) { return b[a].n > b[c].n; });
+}
+
+Buckets bucket_table() {
+    Buckets B;
+    B.count = kNumBuckets;
+    for (int b = 0; b < kNumBuckets; ++b) B.edge[b] = kBucketEdges[b];
+    return B;
+}
+
+int per_cu(const ggx_ctx* ctx, long long lds) {
+    long long k = lds > 0 ? ctx->lds_per_cu / lds : 8;
+    return (int)std::max<long long>(1, std::min<long long>(8, k));
+}
+
+std::vector<hipEvent_t> take_events(ggx_ctx* ctx, int count) {
+    std::vector<hipEvent_t> ev;
+    if (!ctx->ev_free.empty()) {
+        ev = ctx->ev_free.back();
+        ctx->ev_free.pop_back();
+    } else {
+        ev.resize(count);
+        for (auto& e : ev) (void)hipEventCreate(&e);
+    }
+    return ev;
+}
+
+template <class T>
+int launch_step(ggx_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    GridConst G = grid_const(ctx);
+    const int NS = ctx->nslots;
+    T* P = (T*)ctx->P[ctx->cur];
+    T* Pn = (T*)ctx->P[ctx->cur ^ 1];
+    T* Tt = (T*)ctx->Tt;
+    uint32_t* Iold = ctx->I[ctx->icur];
+    uint32_t* Inew = ctx->I[ctx->icur ^ 1];
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;    // dynamic LDS caps
+    const int level_max = ctx->lds_max - ctx->static_level[ti] - 256;
+    const int cus = ctx->cu_count;
+    Sched* S = ctx->sched;
+    std::vector<hipEvent_t> ev = take_events(ctx, 5);
+
+    HIPCHK(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(32), 0, st, S);
+    hipLaunchKernelGGL(k_prep, dim3((NS + 255) / 256), dim3(256), 0, st, NS, ctx->flags, ctx->box, bucket_table(),
+                       ctx->blists, ctx->alist, ctx->errs, S);
+    hipLaunchKernelGGL(k_lists, dim3(cus * 4), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
+                       ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
+    HIPCHK(hipEventRecord(ev[1], st));
+    // grid per bucket: persistent (fixed) or one workgroup per grain (counts read back)
+    int counts[kMaxBuckets];
+    if (!ctx->opt.persistent) {
+        HIPCHK(hipMemcpyAsync(ctx->h_sched, S, sizeof(Sched), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int b = 0; b < kNumBuckets; ++b) counts[b] = ctx->h_sched->bucket_count[b];
+    }
+    // convolution, one launch per bucket
+    for (int b = 0; b < kNumBuckets; ++b) {
+        int nb = kBucketEdges[b];
+        long long kkn = (2LL * nb + 4 + 1) & ~1LL;
+        long long need = (kkn + 2LL * nb * nb) * (long long)sizeof(T);
+        long long lds = std::min<long long>(need, conv_max);
+        if ((kkn + 4LL * nb) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
+        int grid = cus * per_cu(ctx, lds + ctx->static_conv[ti]);
+        if (!ctx->opt.persistent) grid = counts[b];
+        if (grid == 0) continue;
+        hipLaunchKernelGGL(k_conv_circulant<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->blists + (size_t)b * NS,
+                           &S->bucket_count[b], &S->bucket_cursor[b], ctx->box, ctx->off, (const T*)P, Tt,
+                           ctx->kern_off, (const T*)ctx->kern, nb, (int)lds);
+    }
+    HIPCHK(hipEventRecord(ev[2], st));
+    // fused comparison / level set / redistancing, one persistent launch per bucket
+    ContourOut O;
+    O.pts = ctx->pts; O.coff = ctx->coff; O.ccap = ctx->ccap; O.clen = ctx->clen;
+    O.volume = ctx->volume; O.perimeter = ctx->perimeter; O.energy = ctx->energy; O.meanda = ctx->meanda;
+    O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
+    O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
+    O.tops = &S->tops[0]; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
+    for (int b = 0; b < kNumBuckets; ++b) {
+        int nb = kBucketEdges[b];
+        LevelLayout full = level_layout<T>(nb, true), small = level_layout<T>(nb, false);
+        long long lds = full.total <= level_max ? full.total : small.total;
+        if (lds > level_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel");
+        LevelArgs<T> A;
+        A.list = ctx->blists + (size_t)b * NS;
+        A.count = &S->bucket_count[b];
+        A.box = ctx->box; A.off = ctx->off; A.cmp_cnt = ctx->cmp_cnt; A.cmp = ctx->cmp;
+        A.tilde = Tt; A.cglobal = P; A.idl_old = Iold; A.idl_new = Inew;
+        A.off_prev = ctx->off_prev; A.n_prev = ctx->nprev; A.out_field = Pn;
+        A.O = O; A.G = G; A.loop = ctx->loop; A.analysis_timestep = ctx->s.analysis_timestep;
+        A.nb = nb; A.lds_bytes = (int)lds;
+        A.cursor = &S->bucket_cursor2[b];
+        int grid = cus * per_cu(ctx, lds + ctx->static_level[ti]);
+        if (!ctx->opt.persistent) grid = counts[b];
+        if (grid == 0) continue;
+        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(grid), dim3(256), (size_t)lds, st, A);
+        else hipLaunchKernelGGL((k_level<T, false>), dim3(grid), dim3(256), (size_t)lds, st, A);
+    }
+    HIPCHK(hipEventRecord(ev[3], st));
+    HIPCHK(hipGetLastError());
+    ctx->ev_pending.push_back(ev);
+    return GGX_OK;
+}
+
+// Reads the counters of the last step (pinned copy) once the stream is idle.
+int finish(ggx_ctx* ctx) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const Sched& H = *ctx->h_sched;
+    ctx->known_max_n = std::max(ctx->known_max_n, H.max_n);
+    if (H.any_err) {
+        std::vector<int> e((size_t)ctx->nslots);
+        HIPCHK(hipMemcpy(e.data(), ctx->errs, ctx->nslots * sizeof(int), hipMemcpyDeviceToHost));
+        for (int g = 1; g < ctx->nslots; ++g)
+            if (e[g]) {
+                char buf[160];
+                std::snprintf(buf, sizeof buf, "grain %d: device error %d before loop %d", g, e[g], ctx->loop);
+                return fail(ctx, GGX_E_BOUNDS, buf);
+            }
+    }
+    return GGX_OK;
+}
+
+// The per-grain baseline: the same kernels, one launch per grain per kernel, with the work
+// lists built on the host (the reference's per-LSbox calls).
+template <class T>
+int launch_step_pergrain(ggx_ctx* ctx) {
+    hipStream_t st = ctx->stream;
+    GridConst G = grid_const(ctx);
+    const int NS = ctx->nslots;
+    T* P = (T*)ctx->P[ctx->cur];
+    T* Pn = (T*)ctx->P[ctx->cur ^ 1];
+    T* Tt = (T*)ctx->Tt;
+    uint32_t* Iold = ctx->I[ctx->icur];
+    uint32_t* Inew = ctx->I[ctx->icur ^ 1];
+    Sched* S = ctx->sched;
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;
+    const int level_max = ctx->lds_max - ctx->static_level[ti] - 256;
+    std::vector<hipEvent_t> ev = take_events(ctx, 5);
+    // host view of the live grains and their boxes
+    std::vector<int> f((size_t)NS);
+    std::vector<Box> bx((size_t)NS);
+    HIPCHK(hipMemcpy(f.data(), ctx->flags, NS * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bx.data(), ctx->box, NS * sizeof(Box), hipMemcpyDeviceToHost));
+    std::vector<int> live;
+    for (int g = 1; g < NS; ++g)
+        if ((f[g] & kPresent) && (f[g] & kExists)) live.push_back(g);
+    // one-element work lists: grain i at pg_list[i], count 1 at pg_one[i], three zeroed cursors
+    const size_t L = live.size();
+    std::vector<int> ones(L, 1), zeros(3 * L, 0);
+    HIPCHK(hipMemcpy(ctx->pg_list, live.data(), L * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pg_one, ones.data(), L * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pg_cursor, zeros.data(), 3 * L * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(32), 0, st, S);
+    hipLaunchKernelGGL(k_prep, dim3((NS + 255) / 256), dim3(256), 0, st, NS, ctx->flags, ctx->box, bucket_table(),
+                       ctx->blists, ctx->alist, ctx->errs, S);
+    for (size_t i = 0; i < L; ++i)
+        hipLaunchKernelGGL(k_lists, dim3(1), dim3(256), 0, st, ctx->pg_list + i, ctx->pg_one + i, ctx->pg_cursor + i,
+                           ctx->box, ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
+    HIPCHK(hipEventRecord(ev[1], st));
+    for (size_t i = 0; i < live.size(); ++i) {
+        int n = bx[live[i]].n;
+        long long kkn = (2LL * n + 4 + 1) & ~1LL;
+        long long lds = std::min<long long>((kkn + 2LL * n * n) * (long long)sizeof(T), conv_max);
+        if ((kkn + 4LL * n) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
+        hipLaunchKernelGGL(k_conv_circulant<T>, dim3(1), dim3(256), (size_t)lds, st, ctx->pg_list + i, ctx->pg_one + i,
+                           ctx->pg_cursor + L + i, ctx->box, ctx->off, (const T*)P, Tt, ctx->kern_off, (const T*)ctx->kern, n,
+                           (int)lds);
+    }
+    HIPCHK(hipEventRecord(ev[2], st));
+    ContourOut O;
+    O.pts = ctx->pts; O.coff = ctx->coff; O.ccap = ctx->ccap; O.clen = ctx->clen;
+    O.volume = ctx->volume; O.perimeter = ctx->perimeter; O.energy = ctx->energy; O.meanda = ctx->meanda;
+    O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
+    O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
+    O.tops = &S->tops[0]; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
+    for (size_t i = 0; i < live.size(); ++i) {
+        int n = bx[live[i]].n;
+        LevelLayout full = level_layout<T>(n, true), small = level_layout<T>(n, false);
+        long long lds = full.total <= level_max ? full.total : small.total;
+        if (lds > level_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel");
+        LevelArgs<T> A;
+        A.list = ctx->pg_list + i;
+        A.count = ctx->pg_one + i;
+        A.box = ctx->box; A.off = ctx->off; A.cmp_cnt = ctx->cmp_cnt; A.cmp = ctx->cmp;
+        A.tilde = Tt; A.cglobal = P; A.idl_old = Iold; A.idl_new = Inew;
+        A.off_prev = ctx->off_prev; A.n_prev = ctx->nprev; A.out_field = Pn;
+        A.O = O; A.G = G; A.loop = ctx->loop; A.analysis_timestep = ctx->s.analysis_timestep;
+        A.nb = n; A.lds_bytes = (int)lds;
+        A.cursor = ctx->pg_cursor + 2 * L + i;
+        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(1), dim3(256), (size_t)lds, st, A);
+        else hipLaunchKernelGGL((k_level<T, false>), dim3(1), dim3(256), (size_t)lds, st, A);
+    }
+    HIPCHK(hipEventRecord(ev[3], st));
+    HIPCHK(hipGetLastError());
+    ctx->ev_pending.push_back(ev);
+    return GGX_OK;
 }
 
 }  // namespace
@@ -354,6 +442,7 @@ extern "C" int ggx_options_default(ggx_options* o) {
     o->device = 0;
     o->lds_budget = 0;
     o->use_graph = 0;
+    o->persistent = 0;
     o->arena_headroom = 1.5;
     return GGX_OK;
 }
@@ -376,8 +465,6 @@ extern "C" int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx**
     hipError_t e = hipSetDevice(ctx->dev);
     if (e != hipSuccess) { delete ctx; return GGX_E_HIP; }
     if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return GGX_E_HIP; }
-    for (auto& ev : ctx->ev)
-        if (hipEventCreate(&ev) != hipSuccess) { delete ctx; return GGX_E_HIP; }
     if (raise_lds(ctx) != GGX_OK) { delete ctx; return GGX_E_HIP; }
     std::memset(&ctx->st, 0, sizeof(ctx->st));
     ctx->loop = s->start_time;
@@ -394,8 +481,11 @@ extern "C" void ggx_destroy(ggx_ctx* ctx) {
     dfree(ctx->nprev); dfree(ctx->flags); dfree(ctx->errs); dfree(ctx->sec_cnt); dfree(ctx->sec);
     dfree(ctx->cmp_cnt); dfree(ctx->cmp); dfree(ctx->newbox); dfree(ctx->coff); dfree(ctx->ccap); dfree(ctx->clen);
     dfree(ctx->volume); dfree(ctx->perimeter); dfree(ctx->energy); dfree(ctx->meanda); dfree(ctx->nb_cnt);
-    dfree(ctx->nb_id); dfree(ctx->nb_len); dfree(ctx->tops); dfree(ctx->live); dfree(ctx->kern_off); dfree(ctx->kern);
-    for (auto& ev : ctx->ev) if (ev) (void)hipEventDestroy(ev);
+    dfree(ctx->nb_id); dfree(ctx->nb_len); dfree(ctx->sched); dfree(ctx->live); dfree(ctx->kern_off); dfree(ctx->kern);
+    dfree(ctx->blists); dfree(ctx->alist); dfree(ctx->pg_list); dfree(ctx->pg_one); dfree(ctx->pg_cursor);
+    if (ctx->h_sched) (void)hipHostFree(ctx->h_sched);
+    for (auto& v : ctx->ev_pending) for (auto e : v) (void)hipEventDestroy(e);
+    for (auto& v : ctx->ev_free) for (auto e : v) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -551,9 +641,18 @@ extern "C" int ggx_load_network(ggx_ctx* ctx, const ggx_network* net) {
         (rc = dalloc(ctx, ctx->volume, NS)) || (rc = dalloc(ctx, ctx->perimeter, NS)) || (rc = dalloc(ctx, ctx->energy, NS)) ||
         (rc = dalloc(ctx, ctx->meanda, NS)) || (rc = dalloc(ctx, ctx->nb_cnt, NS)) ||
         (rc = dalloc(ctx, ctx->nb_id, (size_t)NS * kNbCap)) || (rc = dalloc(ctx, ctx->nb_len, (size_t)NS * kNbCap)) ||
-        (rc = dalloc(ctx, ctx->tops, 4)) || (rc = dalloc(ctx, ctx->live, NS)) ||
-        (rc = dalloc(ctx, ctx->kern_off, kMaxKernN)))
+        (rc = dalloc(ctx, ctx->sched, 1)) || (rc = dalloc(ctx, ctx->live, NS)) ||
+        (rc = dalloc(ctx, ctx->blists, (size_t)kNumBuckets * NS)) || (rc = dalloc(ctx, ctx->alist, NS)) ||
+        (rc = dalloc(ctx, ctx->pg_list, NS)) || (rc = dalloc(ctx, ctx->pg_one, NS)) ||
+        (rc = dalloc(ctx, ctx->pg_cursor, 3 * (size_t)NS)) || (rc = dalloc(ctx, ctx->kern_off, kMaxKernN)))
         return rc;
+    HIPCHK(hipMemset(ctx->sched, 0, sizeof(Sched)));
+    {
+        void* hp = nullptr;
+        HIPCHK(hipHostMalloc(&hp, sizeof(Sched), 0));
+        ctx->h_sched = (Sched*)hp;
+        std::memset(ctx->h_sched, 0, sizeof(Sched));
+    }
     HIPCHK(hipMalloc(&p, (size_t)kMaxKernN * (kMaxKernN - 1) / 2 * es + 64)); ctx->kern = p;
     ctx->kern_ready.assign(kMaxKernN, 0);
     {
@@ -561,7 +660,8 @@ extern "C" int ggx_load_network(ggx_ctx* ctx, const ggx_network* net) {
         for (int n = 0; n < kMaxKernN; ++n) ko[n] = (long long)n * (n - 1) / 2;
         HIPCHK(hipMemcpy(ctx->kern_off, ko.data(), ko.size() * 8, hipMemcpyHostToDevice));
     }
-    if ((rc = ensure_kernels(ctx, nmax + 16))) return rc;
+    if ((rc = ensure_kernels(ctx, std::min(kMaxKernN - 1, nmax + 32)))) return rc;
+    ctx->known_max_n = nmax;
 
     std::vector<int> hnprev((size_t)NS);
     for (int g = 0; g < NS; ++g) hnprev[g] = hbox[g].n;
@@ -618,62 +718,42 @@ extern "C" int ggx_load_network(ggx_ctx* ctx, const ggx_network* net) {
     return GGX_OK;
 }
 
+// Steps are queued without waiting for the device.  Box sides grow by a few points per step
+// at most, so the circulant table is extended ahead of time and checked every 8 steps.
 extern "C" int ggx_step(ggx_ctx* ctx, int32_t nsteps) {
     if (!ctx) return GGX_E_ARG;
     if (!ctx->loaded) return fail(ctx, GGX_E_STATE, "ggx_step before ggx_load_network");
     for (int s = 0; s < nsteps; ++s) {
-        build_live(ctx);
-        int nlive = (int)ctx->h_live.size();
-        int nmax = 1;
-        long long upd = 0;
-        for (int g : ctx->h_live) { nmax = std::max(nmax, ctx->h_box[g].n); upd += (long long)ctx->h_box[g].n * ctx->h_box[g].n; }
-        int rc = ensure_kernels(ctx, nmax);
-        if (rc) return rc;
-        if (nlive > 0) HIPCHK(hipMemcpyAsync(ctx->live, ctx->h_live.data(), nlive * 4, hipMemcpyHostToDevice, ctx->stream));
-        rc = (ctx->esize == 8) ? launch_step<double>(ctx) : launch_step<float>(ctx);
-        if (rc) return rc;
-        // layout switch: next arena becomes current; this step's IDLocal becomes the stale source
-        HIPCHK(hipMemcpyAsync(ctx->off_prev, ctx->off, ctx->nslots * 8, hipMemcpyDeviceToDevice, ctx->stream));
-        {
-            // n_prev = n of this step's box (gathered through a small copy of box.n)
+        int rc;
+        if (s > 0 && s % 8 == 0) {
+            if ((rc = finish(ctx))) return rc;
         }
+        if ((rc = ensure_kernels(ctx, std::min(kMaxKernN - 1, ctx->known_max_n + 32)))) return rc;
+        if (ctx->opt.batching == GGX_BATCH_PERGRAIN)
+            rc = (ctx->esize == 8) ? launch_step_pergrain<double>(ctx) : launch_step_pergrain<float>(ctx);
+        else
+            rc = (ctx->esize == 8) ? launch_step<double>(ctx) : launch_step<float>(ctx);
+        if (rc) return rc;
+        // layout switch: the next arena becomes current, this step's IDLocal the stale source
         std::swap(ctx->box, ctx->box_next);
         std::swap(ctx->off, ctx->off_next);
         std::swap(ctx->sec, ctx->cmp);
         std::swap(ctx->sec_cnt, ctx->cmp_cnt);
         ctx->cur ^= 1;
         ctx->icur ^= 1;
-        rc = pull_state(ctx);
-        // n_prev for the next step = side of the box this step used
-        {
-            // the previous box array is now box_next; copy its n field on the host
-            std::vector<Box> prevbox((size_t)ctx->nslots);
-            HIPCHK(hipMemcpy(prevbox.data(), ctx->box_next, ctx->nslots * sizeof(Box), hipMemcpyDeviceToHost));
-            std::vector<int> np((size_t)ctx->nslots);
-            for (int g = 0; g < ctx->nslots; ++g) np[g] = prevbox[g].n;
-            HIPCHK(hipMemcpy(ctx->nprev, np.data(), ctx->nslots * 4, hipMemcpyHostToDevice));
-        }
-        float ms[5];
-        for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]));
-        ctx->st.comparison_ms += ms[0] + ms[2];
-        ctx->st.convolution_ms += ms[1];
-        ctx->st.levelset_ms += ms[3];
-        ctx->st.redistancing_ms += ms[4];
-        ctx->st.total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
-        ctx->st.steps += 1;
-        ctx->st.grid_point_updates = upd;
-        ctx->st.live_grains = nlive;
-        ctx->st.max_box = nmax;
         ctx->loop += 1;
-        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(ctx->h_sched, ctx->sched, sizeof(Sched), hipMemcpyDeviceToHost, ctx->stream));
     }
     return GGX_OK;
 }
 
 extern "C" int ggx_sync(ggx_ctx* ctx) {
     if (!ctx) return GGX_E_ARG;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    return GGX_OK;
+    if (!ctx->loaded) {
+        if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+        return GGX_OK;
+    }
+    return finish(ctx);
 }
 
 extern "C" int ggx_loop(const ggx_ctx* ctx) { return ctx ? ctx->loop : GGX_E_ARG; }
@@ -681,14 +761,20 @@ extern "C" int ggx_max_id(const ggx_ctx* ctx) { return ctx ? ctx->max_id : GGX_E
 
 extern "C" int ggx_current_grains(ggx_ctx* ctx) {
     if (!ctx || !ctx->loaded) return GGX_E_STATE;
-    int c = 0;
-    for (int g = 1; g < ctx->nslots; ++g) c += (ctx->h_flags[g] & kPresent) ? 1 : 0;
-    return c;
+    int rc = finish(ctx);
+    if (rc) return rc;
+    if (ctx->h_sched->steps_total == 0) {
+        int c = 0;
+        for (int g = 1; g < ctx->nslots; ++g) c += (ctx->h_flags[g] & kPresent) ? 1 : 0;
+        return c;
+    }
+    return ctx->h_sched->present;
 }
 
 namespace {
 template <class X>
 int fetch(ggx_ctx* ctx, const X* dsrc, X* h, size_t count) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(h, dsrc, count * sizeof(X), hipMemcpyDeviceToHost));
     return GGX_OK;
 }
@@ -820,6 +906,25 @@ extern "C" int ggx_get_compare_list(ggx_ctx* ctx, int32_t id, int32_t* ids, int3
 
 extern "C" int ggx_get_stats(ggx_ctx* ctx, ggx_stats* out) {
     if (!ctx || !out) return GGX_E_ARG;
+    if (ctx->loaded) {
+        int rc = finish(ctx);
+        if (rc) return rc;
+        for (auto& ev : ctx->ev_pending) {
+            float ms[3];
+            for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+            ctx->st.comparison_ms += ms[0];   // scheduling + comparison lists
+            ctx->st.convolution_ms += ms[1];
+            ctx->st.levelset_ms += ms[2];     // fused comparison / level set / redistancing
+            ctx->st.total_ms += ms[0] + ms[1] + ms[2];
+            ctx->st.steps += 1;
+            ctx->ev_free.push_back(ev);
+        }
+        ctx->ev_pending.clear();
+        const Sched& H = *ctx->h_sched;
+        ctx->st.grid_point_updates = (int64_t)H.updates_total;
+        ctx->st.live_grains = H.all_count;
+        ctx->st.max_box = H.max_n;
+    }
     *out = ctx->st;
     return GGX_OK;
 }

@@ -1,0 +1,679 @@
+// level.h -- the fused level-set kernel: comparison, contour, neighbours and redistancing of
+// one grain per workgroup, with the compared field and the redistanced tile resident in LDS.
+//
+// It performs, for every grain of a size bucket, what the reference does in the phases
+// comparison_box, level_set and redistancing (src/grainhdl.cpp:815-869):
+//   LSbox::executeComparison (box.cpp:890-943) + boundaryCondition (968-1025)
+//   LSbox::executeSetComparison (838-879)
+//   LSbox::extractContour (1089-1153): MarchingSquaresAlgorithm::generateContour
+//     (marchingSquares.cpp:28-188), computeVolume, isMotionRegular (1155-1190),
+//     clearContourGrainArea (1883-1904), bounds clamp + resizeToSquare, computePerimeter
+//   LSbox::computeVolumeAndEnergy (1201-1228): buildDirectNeighbours (grainBoundary.cpp:85-199)
+//   LSbox::executeRedistancing (1259-1399)
+// The compared field never leaves the chip.  The trace is split: lane 0 walks the squares using
+// only inside/outside tests and records which edge each point lies on; all lanes then
+// interpolate the points, drop consecutive duplicates with a block scan and write the contour.
+#pragma once
+#include "kernels.h"
+
+namespace ggx {
+
+template <class T>
+struct LevelArgs {
+    const int* list;           // grains of this bucket
+    const int* count;          // number of grains in the bucket (device)
+    int* cursor;               // work-list cursor (device, zero at launch)
+    const Box* box;
+    const long long* off;
+    const int* cmp_cnt;
+    const int* cmp;
+    const T* tilde;            // convolved fields (all grains)
+    T* cglobal;                // phi arena of this step: holds C when it does not fit in LDS
+    const uint32_t* idl_old;
+    uint32_t* idl_new;
+    long long* off_prev;       // in: IDLocal layout of the previous step; out: this step's
+    int* n_prev;
+    T* out_field;              // next phi arena
+    ContourOut O;
+    GridConst G;
+    int loop;
+    int analysis_timestep;
+    int nb;                    // largest box side of the bucket (sizes the LDS regions)
+    int lds_bytes;
+};
+
+// Block-wide exclusive scan of one int per lane (blockDim.x <= 1024); returns the total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
+    int t = threadIdx.x;
+    s_tmp[t] = v;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+        int add = (t >= d) ? s_tmp[t - d] : 0;
+        __syncthreads();
+        s_tmp[t] += add;
+        __syncthreads();
+    }
+    int incl = s_tmp[t];
+    __syncthreads();
+    return incl - v;
+}
+
+// Interpolated point of a recorded move (marchingSquares.cpp:204-265).
+template <class T>
+__device__ __forceinline__ double2 move_point(const T* C, int n, int x0, int y0, uint32_t mv) {
+    int dir = (int)(mv & 3u);
+    int left = (int)((mv >> 2) & 0x3fffu) + x0 - 1;
+    int top = (int)((mv >> 16) & 0x3fffu) + y0 - 1;
+    int right = left + 1, bottom = top + 1;
+    auto val = [&](int row, int col) { return (double)C[(row - y0) * n + (col - x0)]; };
+    double2 r;
+    double delta;
+    switch (dir) {
+        case D_LEFT:
+            r.x = left;
+            if (top < y0) r.y = bottom;
+            else { delta = val(bottom, left) - val(top, left); r.y = bottom - val(bottom, left) / delta; }
+            break;
+        case D_RIGHT:
+            r.x = right;
+            if (bottom >= y0 + n) r.y = top;
+            else { delta = val(top, right) - val(bottom, right); r.y = top + val(top, right) / delta; }
+            break;
+        case D_UP:
+            r.y = top;
+            if (right >= x0 + n) r.x = left;
+            else { delta = val(top, left) - val(top, right); r.x = left + val(top, left) / delta; }
+            break;
+        default:  // D_DOWN
+            r.y = bottom;
+            if (left < x0) r.x = right;
+            else { delta = val(bottom, right) - val(bottom, left); r.x = right - val(bottom, right) / delta; }
+            break;
+    }
+    return r;
+}
+
+// LDS layout of the fused kernel for boxes up to side nb.  Full layout: C tile, O tile (which
+// also hosts the per-point work arrays until redistancing), moves, scan scratch.  Small
+// layout (boxes too large for LDS): C and O stay in global memory, only the work arrays,
+// moves and scan scratch are in LDS.
+struct LevelLayout {
+    long long ctile, otile, moves, scan, total;
+};
+
+template <class T>
+__host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
+    LevelLayout L;
+    long long ccap = contour_cap(nb);
+    long long no = nb + 8;
+    long long work = (ccap * 20 + 15) & ~15LL;
+    L.ctile = full ? (((long long)nb * nb * (long long)sizeof(T) + 15) & ~15LL) : 0;
+    L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
+    if (work > L.otile) L.otile = work;
+    L.moves = (ccap * 4 + 15) & ~15LL;
+    L.scan = 1024 * 4;
+    L.total = L.ctile + L.otile + L.moves + L.scan;
+    return L;
+}
+
+// LSbox::executeRedistancing (box.cpp:1259-1399) of one grain: Ob is the output tile (LDS with
+// an odd pitch, or the global slot itself), C the compared field on the old box b, bo the new
+// box.  Lanes own columns (y sweeps) or rows (x sweeps); the value carried along a sweep stays
+// in a register.
+template <class T>
+__device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, Box b, Box bo, const GridConst& G,
+                                           T* gout) {
+    const int n = b.n, no = bo.n;
+    const double h = G.h;
+    const int oxmin = bo.x0, oxmax = bo.x0 + no, oymin = bo.y0, oymax = bo.y0 + no;
+    const int ixmn = b.x0, ixmx = b.x0 + n, iymn = b.y0, iymx = b.y0 + n;
+    const int ixmin = ixmn < oxmin ? oxmin : ixmn;
+    const int iymin = iymn < oymin ? oymin : iymn;
+    const int ixmax = ixmx < oxmax ? ixmx : oxmax;
+    const int iymax = iymx < oymax ? iymx : oymax;
+#define OV(i, j) Ob[((i) - oymin) * pitch + ((j) - oxmin)]
+#define CI(i, j) C[((i) - iymn) * n + ((j) - ixmn)]
+    for (int p = threadIdx.x; p < no * no; p += blockDim.x) {
+        int yy = p / no, xx = p - yy * no;
+        Ob[yy * pitch + xx] = T(-1.0);
+    }
+    __syncthreads();
+    // y forward (box.cpp:1291-1319)
+    for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
+        if (iymin >= oymax - 1) break;
+        T cur = OV(iymin, j);
+        for (int i = iymin; i < oymax - 1; i++) {
+            T nxt = OV(i + 1, j);
+            double cand;
+            if (j < ixmax && i < iymax - 1) {
+                double a = (double)CI(i, j), an = (double)CI(i + 1, j);
+                if (a * an <= 0.0) {
+                    double slope = (an - a) / h;
+                    double d0 = -a / slope;
+                    if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
+                }
+                cand = (double)cur + (sgn_of(CI(i + 1, j)) * h);
+            } else {
+                cand = (double)cur + (sgn_of(nxt) * h);
+            }
+            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i + 1, j) = nxt; }
+            cur = nxt;
+        }
+    }
+    __syncthreads();
+    // y backward (1321-1338)
+    for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
+        if (iymax - 1 <= oymin) break;
+        T cur = OV(iymax - 1, j);
+        for (int i = iymax - 1; i > oymin; i--) {
+            T nxt = OV(i - 1, j);
+            double cand;
+            if (j < ixmax && i > iymin) cand = (double)cur + (sgn_of(CI(i - 1, j)) * h);
+            else cand = (double)cur + (sgn_of(nxt) * h);
+            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i - 1, j) = nxt; }
+            cur = nxt;
+        }
+    }
+    __syncthreads();
+    // x forward (1339-1367)
+    for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
+        if (ixmin >= oxmax - 1) break;
+        T cur = OV(i, ixmin);
+        for (int j = ixmin; j < oxmax - 1; j++) {
+            T nxt = OV(i, j + 1);
+            double cand;
+            if (j < ixmax - 1 && i < iymax) {
+                double a = (double)CI(i, j), an = (double)CI(i, j + 1);
+                if (a * an <= 0.0) {
+                    double slope = (an - a) / h;
+                    double d0 = -a / slope;
+                    if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
+                }
+                cand = (double)cur + (sgn_of(CI(i, j + 1)) * h);
+            } else {
+                cand = (double)cur + (sgn_of(nxt) * h);
+            }
+            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j + 1) = nxt; }
+            cur = nxt;
+        }
+    }
+    __syncthreads();
+    // x backward (1369-1388)
+    for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
+        if (ixmax - 1 <= oxmin) break;
+        T cur = OV(i, ixmax - 1);
+        for (int j = ixmax - 1; j > oxmin; j--) {
+            T nxt = OV(i, j - 1);
+            double cand;
+            if (j > ixmin && i < iymax) cand = (double)cur + (sgn_of(CI(i, j - 1)) * h);
+            else cand = (double)cur + sgn_of(nxt) * h;
+            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j - 1) = nxt; }
+            cur = nxt;
+        }
+    }
+    __syncthreads();
+#undef OV
+#undef CI
+    // clampValues (dimensionalBufferReal.h:43-56)
+    const T lo = (T)(-G.delta), hi = (T)G.delta;
+    for (int p = threadIdx.x; p < no * no; p += blockDim.x) {
+        int yy = p / no, xx = p - yy * no;
+        T v = Ob[yy * pitch + xx];
+        if (v < lo) v = lo;
+        else if (v > hi) v = hi;
+        gout[p] = v;
+    }
+}
+
+template <class T, bool FULL>
+__global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
+    HIP_DYNAMIC_SHARED(unsigned char, smem)
+    const GridConst& G = A.G;
+    const int nb = A.nb;
+    const int ccap = contour_cap(nb);
+    const LevelLayout LY = level_layout<T>(nb, FULL);
+    T* Cl = (T*)smem;
+    unsigned char* obase = smem + LY.ctile;
+    T* Ol = (T*)obase;
+    double* s_len = (double*)obase;                       // aliases O until redistancing
+    double* s_term = s_len + ccap;
+    int* s_cand = (int*)(s_term + ccap);
+    uint32_t* moves = (uint32_t*)(obase + LY.otile);
+    int* s_scan = (int*)((unsigned char*)moves + LY.moves);
+
+    __shared__ int s_bb[4];
+    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len, 7 stop
+    __shared__ double s_dbl[8];   // 0 volume sum, 1 minx, 2 maxx, 3 miny, 4 maxy
+    __shared__ int s_id[kCmpCap], s_x0[kCmpCap], s_y0[kCmpCap], s_n[kCmpCap];
+    __shared__ long long s_off[kCmpCap];
+    __shared__ long long s_coff, s_onext;
+    __shared__ Box s_nbx;
+
+    __shared__ int s_slot;
+    while (true) {
+        const int item = next_item(A.cursor, &s_slot);
+        if (item >= *A.count) break;
+        const int g = A.list[item];
+        const Box b = A.box[g];
+        const int n = b.n;
+        const long long o = A.off[g];
+        const long long nn = (long long)n * n;
+        T* C;
+        if constexpr (FULL) C = Cl;
+        else C = A.cglobal + o;
+
+        // ---------------- comparison + set-comparison ----------------
+        const int cnt = A.cmp_cnt[g];
+        for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+            int nbg = A.cmp[(size_t)g * kCmpCap + k];
+            Box bn = A.box[nbg];
+            s_id[k] = nbg;
+            s_x0[k] = bn.x0; s_y0[k] = bn.y0; s_n[k] = bn.n;
+            s_off[k] = A.off[nbg];
+        }
+        if (threadIdx.x == 0) {
+            s_bb[0] = b.y0 + n; s_bb[1] = b.y0; s_bb[2] = b.x0 + n; s_bb[3] = b.x0;
+            s_int[3] = 0; s_int[4] = 1; s_int[5] = 1;
+            unsigned long long co = atomicAdd(&A.O.tops[1], (unsigned long long)ccap);
+            s_coff = (long long)co;
+            if (co + (unsigned long long)ccap > (unsigned long long)A.O.contour_cap) s_int[3] = kErrContourCap;
+        }
+        const long long np = (long long)A.n_prev[g] * A.n_prev[g];
+        const long long op = A.off_prev[g];
+        __syncthreads();
+        {
+            const int lo = 2 * G.gb, hi = G.m - 2 * G.gb;
+            const bool bnd = !(b.x0 > lo && b.x0 + n < hi && b.y0 > lo && b.y0 + n < hi);
+            const double band = 0.7 * G.delta;
+            int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
+            for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
+                int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
+                int i = b.y0 + yy, j = b.x0 + xx;
+                T mx = T(-1.0);
+                int id = -1;
+                for (int k = 0; k < cnt; ++k) {
+                    int ly = i - s_y0[k], lx = j - s_x0[k];
+                    int nk = s_n[k];
+                    if ((unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk) {
+                        T v = A.tilde[s_off[k] + (long long)ly * nk + lx];
+                        if (v > mx) { mx = v; id = s_id[k]; }
+                    }
+                }
+                if (bnd) {
+                    int gb = G.gb, m = G.m;
+                    int dXMin = -(j - gb), dYMin = -(i - gb);
+                    int dXMax = (j - (m - gb)), dYMax = (i - (m - gb));
+                    int dX = (abs(dXMin) < abs(dXMax)) ? dXMin : dXMax;
+                    int dY = (abs(dYMin) < abs(dYMax)) ? dYMin : dYMax;
+                    double dist;
+                    if (dX > 0 && dY > 0) dist = sqrt((double)dX * dX + dY * dY);
+                    else if (dX < 0 && dY > 0) dist = dY;
+                    else if (dX > 0 && dY < 0) dist = dX;
+                    else if (dX < 0 && dY < 0) dist = max(dX, dY);
+                    else if (dX == 0) dist = (dY > 0) ? (double)dY : 0.0;
+                    else dist = (dX > 0) ? (double)dX : 0.0;
+                    if ((T)(dist * G.h) > mx) { mx = (T)(dist * G.h); id = 0; }
+                }
+                uint32_t idv = (id >= 0) ? (uint32_t)id : ((p < np) ? A.idl_old[op + p] : 0u);
+                A.idl_new[o + p] = idv;
+                T t = A.tilde[o + p];
+                T c = (fabs((double)t) < band) ? (T)(0.5 * ((double)t - (double)mx)) : t;
+                C[p] = c;
+                if (c >= T(0)) {
+                    ymin_l = min(ymin_l, i); ymax_l = max(ymax_l, i);
+                    xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
+                }
+            }
+            if (ymin_l != 0x7fffffff) {
+                atomicMin(&s_bb[0], ymin_l); atomicMax(&s_bb[1], ymax_l);
+                atomicMin(&s_bb[2], xmin_l); atomicMax(&s_bb[3], xmax_l);
+            }
+            if (threadIdx.x == 0) {
+                int f = A.O.flags[g];
+                A.O.flags[g] = bnd ? (f | kBoundary) : (f & ~kBoundary);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) {  // this step's IDLocal is the stale source of the next step
+            A.n_prev[g] = n;
+            A.off_prev[g] = o;
+        }
+        if (s_int[3]) {
+            if (threadIdx.x == 0) A.O.err[g] = s_int[3];
+            __syncthreads();
+            continue;
+        }
+        double2* pts = A.O.pts + s_coff;
+        const int old_len = A.O.clen[g];
+        const double old_vol = A.O.volume[g];
+
+        // ---------------- contour with motion check and retries ----------------
+        int retries = 0;
+        while (true) {
+            if (threadIdx.x == 0) s_int[0] = 0x7fffffff;
+            __syncthreads();
+            {
+                int loc = 0x7fffffff;
+                for (long long p = threadIdx.x; p < nn; p += blockDim.x)
+                    if (C[p] >= T(0)) { loc = (int)p; break; }
+                if (loc != 0x7fffffff) atomicMin(&s_int[0], loc);
+            }
+            __syncthreads();
+            if (s_int[0] == 0x7fffffff) {  // no inside point: the grain vanished
+                if (threadIdx.x == 0) { s_int[5] = 0; s_int[6] = 0; }
+                __syncthreads();
+                break;
+            }
+            // lane 0 walks the squares, recording point-generating moves
+            if (threadIdx.x == 0) {
+                const int sx = b.x0 + s_int[0] % n, sy = b.y0 + s_int[0] / n;
+                auto inside = [&](int row, int col) -> bool {
+                    int r = row - b.y0, q = col - b.x0;
+                    if ((unsigned)r >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
+                    return C[r * n + q] >= T(0);
+                };
+                int left = sx - 1, top = sy - 1;
+                int next = D_NONE, prev = D_NONE;
+                long long itcap = 8LL * (n + 2) * (n + 2) + 16, it = 0;
+                int nm = 0, st = 0;
+                bool tl = inside(top, left), tr = inside(top, left + 1);
+                bool bl = inside(top + 1, left), br = inside(top + 1, left + 1);
+                do {
+                    int mask = (tl ? MS_TL : 0) | (tr ? MS_TR : 0) | (bl ? MS_BL : 0) | (br ? MS_BR : 0);
+                    bool gen = true;
+                    switch (mask) {
+                        case MS_TL: next = D_UP; break;
+                        case MS_TR: next = D_RIGHT; break;
+                        case MS_TL | MS_TR: next = D_RIGHT; break;
+                        case MS_BL: next = D_LEFT; break;
+                        case MS_BL | MS_TL: next = D_UP; break;
+                        case MS_BL | MS_TR: next = (prev == D_UP) ? D_RIGHT : D_LEFT; gen = false; break;
+                        case MS_BL | MS_TR | MS_TL: next = D_RIGHT; break;
+                        case MS_BR: next = D_DOWN; break;
+                        case MS_BR | MS_TL: next = (prev == D_RIGHT) ? D_DOWN : D_UP; gen = false; break;
+                        case MS_BR | MS_TR: next = D_DOWN; break;
+                        case MS_BR | MS_TR | MS_TL: next = D_DOWN; break;
+                        case MS_BR | MS_BL: next = D_LEFT; break;
+                        case MS_BR | MS_BL | MS_TL: next = D_UP; break;
+                        case MS_BR | MS_BL | MS_TR: next = D_LEFT; break;
+                        default: gen = false; break;
+                    }
+                    if (gen) {
+                        if (nm >= ccap) { st = kErrContourCap; break; }
+                        moves[nm++] = (uint32_t)next | ((uint32_t)(left - b.x0 + 1) << 2) | ((uint32_t)(top - b.y0 + 1) << 16);
+                    }
+                    switch (next) {  // move and reuse the two shared corners
+                        case D_UP: top--; bl = tl; br = tr; tl = inside(top, left); tr = inside(top, left + 1); break;
+                        case D_DOWN: top++; tl = bl; tr = br; bl = inside(top + 1, left); br = inside(top + 1, left + 1); break;
+                        case D_LEFT: left--; tr = tl; br = bl; tl = inside(top, left); bl = inside(top + 1, left); break;
+                        case D_RIGHT: left++; tl = tr; bl = br; tr = inside(top, left + 1); br = inside(top + 1, left + 1); break;
+                        default: break;
+                    }
+                    prev = next;
+                    if (++it > itcap) { st = kErrTrace; break; }
+                } while (left != sx - 1 || top + 1 != sy);
+                s_int[1] = nm;
+                if (st) s_int[3] = st;
+            }
+            __syncthreads();
+            if (s_int[3]) {
+                if (threadIdx.x == 0) s_int[5] = 0;
+                __syncthreads();
+                break;
+            }
+            // all lanes: interpolate, drop consecutive duplicates (insertPoint,
+            // marchingSquares.cpp:197-203), compact in order
+            const int nm = s_int[1];
+            const int per = (nm + blockDim.x - 1) / blockDim.x;
+            const int k0 = threadIdx.x * per, k1 = min(nm, k0 + per);
+            int keep_cnt = 0;
+            for (int k = k0; k < k1; ++k) {
+                double2 p = move_point(C, n, b.x0, b.y0, moves[k]);
+                bool keep = true;
+                if (k > 0) {
+                    double2 q = move_point(C, n, b.x0, b.y0, moves[k - 1]);
+                    keep = (p.x != q.x || p.y != q.y);
+                }
+                keep_cnt += keep ? 1 : 0;
+            }
+            int base = block_exclusive_scan(keep_cnt, s_scan);
+            int w = base;
+            for (int k = k0; k < k1; ++k) {
+                double2 p = move_point(C, n, b.x0, b.y0, moves[k]);
+                bool keep = true;
+                if (k > 0) {
+                    double2 q = move_point(C, n, b.x0, b.y0, moves[k - 1]);
+                    keep = (p.x != q.x || p.y != q.y);
+                }
+                if (keep) {
+                    if (w < ccap) pts[w] = p;
+                    w++;
+                }
+            }
+            if (k1 == nm && k0 < k1) s_int[6] = w;  // lane holding the last move knows the count
+            if (nm == 0 && threadIdx.x == 0) s_int[6] = 0;
+            __threadfence_block();
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int len = s_int[6];
+                if (len > 0) {
+                    double2 f = pts[0], l = pts[len - 1];
+                    if (f.x != l.x || f.y != l.y) {
+                        if (len < ccap) pts[len] = f;
+                        len++;
+                    }
+                }
+                if (len > ccap) s_int[3] = kErrContourCap;
+                s_int[6] = len;
+            }
+            __threadfence_block();
+            __syncthreads();
+            const int len = s_int[6];
+            if (s_int[3]) {
+                if (threadIdx.x == 0) s_int[5] = 0;
+                __syncthreads();
+                break;
+            }
+            // shoelace terms and segment lengths, summed in order by lane 0
+            for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
+                double2 a = pts[i], c2 = pts[i + 1];
+                s_term[i] = (a.y + c2.y) * (a.x - c2.x);
+                double dx = a.x - c2.x, dy = a.y - c2.y;
+                s_len[i] = sqrt(dx * dx + dy * dy);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double vs = 0;
+                for (int i = 0; i < len - 1; ++i) vs += s_term[i];
+                s_dbl[0] = vs;
+                double newVolume = fabs(vs) * G.h * G.h * 0.5;
+                double mullins = (newVolume - old_vol) / G.dt * (3 / 3.14159265358979323846);
+                double contourRatio = ((double)len) / old_len;
+                double volumeRatio = newVolume / old_vol;
+                int regular;
+                if (mullins > -6.0) regular = 1;
+                else if (volumeRatio < 0.10) regular = 0;
+                else if (contourRatio < 0.35) regular = 0;
+                else if (contourRatio > 0.7) regular = 1;
+                else { s_int[4] = 0; regular = 1; }
+                s_int[2] = regular ? 2 : 1;
+                if (!regular) {  // bounding box of the contour for clearContourGrainArea
+                    double mnx = pts[0].x, mxx = mnx, mny = pts[0].y, mxy = mny;
+                    for (int i = 0; i < len; ++i) {
+                        mnx = fmin(mnx, pts[i].x); mxx = fmax(mxx, pts[i].x);
+                        mny = fmin(mny, pts[i].y); mxy = fmax(mxy, pts[i].y);
+                    }
+                    s_dbl[1] = mnx; s_dbl[2] = mxx; s_dbl[3] = mny; s_dbl[4] = mxy;
+                }
+            }
+            __syncthreads();
+            if (s_int[2] == 2) break;
+            if (++retries > 256) {
+                if (threadIdx.x == 0) { s_int[3] = kErrTrace; s_int[5] = 0; }
+                __syncthreads();
+                break;
+            }
+            {  // clearContourGrainArea: i in [int(miny-1), maxy+1), j in [int(minx-1), maxx+1)
+                int i0 = (int)(s_dbl[3] - 1), j0 = (int)(s_dbl[1] - 1);
+                double maxy = s_dbl[4], maxx = s_dbl[2];
+                int ih = 0, jw = 0;
+                while (i0 + ih < maxy + 1) ih++;
+                while (j0 + jw < maxx + 1) jw++;
+                int bad = 0;
+                for (int p = threadIdx.x; p < ih * jw; p += blockDim.x) {
+                    int ii = i0 + p / jw, jj = j0 + p % jw;
+                    long long k = flat_index(ii, jj, b.x0, b.y0, n, nn);
+                    if (k < 0) { bad = 1; continue; }
+                    if (C[k] >= T(0)) C[k] = (T)(-G.h);
+                }
+                if (bad) atomicOr(&s_int[3], kErrClearBounds);
+                __threadfence_block();
+                __syncthreads();
+                if (s_int[3]) {
+                    if (threadIdx.x == 0) s_int[5] = 0;
+                    __syncthreads();
+                    break;
+                }
+            }
+        }
+        const int exists = s_int[5] && !s_int[3];
+        const int len = s_int[6];
+        if (threadIdx.x == 0) {
+            int f = A.O.flags[g];
+            f = s_int[4] ? (f | kRegular) : (f & ~kRegular);
+            if (!exists) f &= ~kExists;
+            A.O.flags[g] = f;
+            if (s_int[3]) A.O.err[g] = s_int[3];
+            A.O.box_next[g] = b;
+            A.O.coff[g] = s_coff;
+            A.O.ccap[g] = ccap;
+            if (exists) A.O.clen[g] = len;
+        }
+        if (!exists) {
+            __syncthreads();
+            continue;
+        }
+        // ---------------- new box (box.cpp:1118-1148) ----------------
+        if (threadIdx.x == 0) {
+            int nXMin = s_bb[2] - G.gb, nXMax = s_bb[3] + G.gb, nYMin = s_bb[0] - G.gb, nYMax = s_bb[1] + G.gb;
+            if (nXMin >= nXMax || nYMin >= nYMax) { nXMin = b.x0; nXMax = b.x0 + n; nYMin = b.y0; nYMax = b.y0 + n; }
+            int m = G.m;
+            if (nXMin < 0) nXMin = 0;
+            if (nYMin < 0) nYMin = 0;
+            if (nXMax > m) nXMax = m;
+            if (nYMax > m) nYMax = m;
+            int xmin = nXMin, xmax = nXMax, ymin = nYMin, ymax = nYMax;
+            int grid_size = m - 1;
+            unsigned int height = (unsigned int)(ymax - ymin), width = (unsigned int)(xmax - xmin);
+            if (width != height) {
+                if (width > height) { int diff = (int)(width - height); ymin -= diff / 2; ymax += diff / 2 + diff % 2; }
+                else { int diff = (int)(height - width); xmin -= diff / 2; xmax += diff / 2 + diff % 2; }
+                if (xmin < 0) { int dl = -xmin; xmax += dl; xmin += dl; }
+                else if (xmax > grid_size) { int dl = grid_size - xmax; xmax += dl; xmin += dl; }
+                if (ymin < 0) { int dl = -ymin; ymax += dl; ymin += dl; }
+                else if (ymax > grid_size) { int dl = grid_size - ymax; ymax += dl; ymin += dl; }
+            }
+            Box nbx;
+            nbx.x0 = xmin; nbx.y0 = ymin; nbx.n = xmax - xmin; nbx.pad = ymax - ymin;
+            long long need = ((long long)nbx.n * nbx.n + kAlign - 1) / kAlign * kAlign;
+            unsigned long long fo = atomicAdd(&A.O.tops[0], (unsigned long long)need);
+            if ((long long)(fo + need) > A.O.field_cap) A.O.err[g] = kErrArena;
+            A.O.box_next[g] = nbx;
+            A.O.off_next[g] = (long long)fo;
+            s_nbx = nbx;
+            s_onext = (long long)fo;
+            double per = 0;  // computePerimeter (grainBoundary.cpp:224-231)
+            for (int i = 0; i < len - 1; ++i) per += s_len[i] * G.h;
+            A.O.perimeter[g] = per;
+        }
+        __syncthreads();
+        // ---------------- direct neighbours (only after a regular motion) ----------------
+        if (s_int[4]) {
+            for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
+                double2 p = pts[i];
+                int pxGrid = (int)p.x, pyGrid = (int)p.y;
+                int cand = -1;
+                bool ok = true;
+                long long k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
+                if (k < 0) ok = false;
+                else if (C[k] > T(0)) {
+                    pxGrid = (int)(p.x + 1);
+                    k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
+                    if (k < 0) ok = false;
+                    else if (C[k] > T(0)) {
+                        pyGrid = (int)(p.y + 1);
+                        k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
+                        if (k < 0) ok = false;
+                        else if (C[k] > T(0)) pxGrid = (int)p.x;
+                    }
+                }
+                if (ok) {
+                    if (pyGrid < 0 || pyGrid >= b.y0 + n || pxGrid < 0 || pxGrid >= b.x0 + n) ok = false;
+                    else {
+                        k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
+                        if (k < 0) ok = false;
+                        else {
+                            uint32_t nid = A.idl_new[o + k];
+                            if (nid == 0u) cand = 0;
+                            else if (nid < (uint32_t)G.nslots && (A.O.flags[nid] & kPresent)) cand = (int)nid;
+                            else ok = false;
+                        }
+                    }
+                }
+                s_cand[i] = ok ? cand : -1;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int cnt2 = 0;
+                int* ids = A.O.nb_id + (size_t)g * kNbCap;
+                double* lens = A.O.nb_len + (size_t)g * kNbCap;
+                int stop = len - 1, over = 0;
+                for (int i = 0; i < len - 1; ++i) {
+                    int cd = s_cand[i];
+                    if (cd < 0) { stop = i; break; }
+                    double ll = s_len[i] * G.h;
+                    int e = 0;
+                    for (; e < cnt2; ++e)
+                        if (ids[e] == cd) break;
+                    if (e == cnt2) {
+                        if (cnt2 < kNbCap) { ids[cnt2] = cd; lens[cnt2] = 0.0; cnt2++; }
+                        else { over = 1; e = -1; }
+                    }
+                    if (e >= 0) lens[e] += ll;
+                }
+                A.O.nb_cnt[g] = cnt2;
+                if (over) A.O.err[g] = kErrNbCap;
+                if (stop < len - 1) A.O.flags[g] &= ~kRegular;  // markAsInvalidMotion
+                double e = 0;
+                for (int i = 0; i < len - 1; ++i) e += s_len[i] * (i < stop ? 1.0 : -1.0) * G.h;
+                A.O.energy[g] = e;
+                double newVolume = fabs(fabs(s_dbl[0]) * G.h * G.h * 0.5);
+                double dA = newVolume - old_vol;
+                dA = dA / G.dt;
+                dA = dA * (3 / 3.14159265358979323846);
+                A.O.volume[g] = newVolume;
+                unsigned int lp = (unsigned int)A.loop;
+                unsigned long at = A.analysis_timestep > 0 ? (unsigned long)A.analysis_timestep : 1ul;
+                double md = A.O.meanda[g];
+                if ((unsigned long)(lp - 1u) % at == 0 || lp == 0) md = 0;
+                A.O.meanda[g] = md + dA;
+            }
+        }
+        __syncthreads();
+        // ---------------- redistancing ----------------
+        {
+            const Box bo = s_nbx;
+            T* gout = A.out_field + s_onext;
+            if constexpr (FULL) {
+                if (bo.n <= nb + 8) redistance<T>(Ol, bo.n | 1, C, b, bo, G, gout);
+                else redistance<T>(gout, bo.n, C, b, bo, G, gout);
+            } else {
+                redistance<T>(gout, bo.n, C, b, bo, G, gout);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace ggx

@@ -140,6 +140,8 @@ typedef struct ggx_options {
     int32_t device;           /* HIP device ordinal */
     int32_t lds_budget;       /* bytes of LDS a workgroup may use; 0 = device maximum */
     int32_t use_graph;        /* capture the step in a hipGraph */
+    int32_t persistent;       /* 1: fixed grids pull grains from device work lists (no host sync);
+                                 0: one workgroup per grain, counts read back after scheduling */
     double  arena_headroom;   /* arena capacity = headroom * initial sum of box areas */
 } ggx_options;
 int ggx_options_default(ggx_options* o);

@@ -50,7 +50,7 @@ inline std::vector<char*> g_stacks;
 inline int g_cur = 0;
 inline dim3 g_tid, g_bid, g_bdim, g_gdim;
 inline std::function<void()>* g_body = nullptr;
-inline std::vector<unsigned char> g_dyn;
+inline unsigned char* g_dyn = nullptr;
 constexpr size_t kStack = 256 * 1024;
 
 inline void entry() {
@@ -69,7 +69,9 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> bo
     g_bdim = block;
     for (unsigned b = 0; b < grid.x; ++b) {
         g_bid = dim3(b, 0, 0);
-        g_dyn.assign(shmem + 16, 0xCD);
+        std::free(g_dyn);
+        g_dyn = (unsigned char*)std::malloc(shmem ? shmem : 1);
+        std::memset(g_dyn, 0xCD, shmem ? shmem : 1);
         for (unsigned t = 0; t < nt; ++t) {
             getcontext(&g_fibers[t].ctx);
             g_fibers[t].ctx.uc_stack.ss_sp = g_stacks[t];
@@ -98,7 +100,7 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> bo
 #define blockIdx (emu::g_bid)
 #define blockDim (emu::g_bdim)
 #define gridDim (emu::g_gdim)
-#define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)emu::g_dyn.data();
+#define HIP_DYNAMIC_SHARED(type, var) type* var = (type*)emu::g_dyn;
 inline void __syncthreads() { emu::sync(); }
 inline void __threadfence() {}
 inline void __threadfence_block() {}
@@ -108,6 +110,7 @@ inline int atomicAdd(int* p, int v) { int o = *p; *p += v; return o; }
 inline int atomicMin(int* p, int v) { int o = *p; if (v < o) *p = v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 inline int atomicOr(int* p, int v) { int o = *p; *p |= v; return o; }
+inline int atomicCAS(int* p, int cmp, int v) { int o = *p; if (o == cmp) *p = v; return o; }
 inline unsigned atomicAnd(unsigned* p, unsigned v) { unsigned o = *p; *p &= v; return o; }
 
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
@@ -128,6 +131,8 @@ struct hipDeviceProp_t {
 inline const char* hipGetErrorString(hipError_t) { return "emu error"; }
 inline hipError_t hipMalloc(void** p, size_t n) { *p = std::calloc(n ? n : 1, 1); return *p ? 0 : 2; }
 inline hipError_t hipFree(void* p) { std::free(p); return 0; }
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { *p = std::calloc(n ? n : 1, 1); return *p ? 0 : 2; }
+inline hipError_t hipHostFree(void* p) { std::free(p); return 0; }
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return 0; }
 inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
@@ -151,3 +156,7 @@ inline hipError_t hipGetDeviceProperties(hipDeviceProp_t* p, int) {
 }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
 inline hipError_t hipGetLastError() { return 0; }
+struct hipFuncAttributes {
+    size_t sharedSizeBytes;
+};
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { a->sharedSizeBytes = 0; return 0; }

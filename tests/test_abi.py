@@ -28,7 +28,7 @@ def test_struct_sizes_match_c_layout(lib):
     # ggx_settings: 27 int32 + 11 doubles + 2 * 512 chars; natural alignment
     assert ctypes.sizeof(_native.Settings) == 27 * 4 + 4 + 11 * 8 + 1024
     assert ctypes.sizeof(_native.GrainInfo) == 16 * 4 + 4 * 8
-    assert ctypes.sizeof(_native.Options) == 6 * 4 + 8
+    assert ctypes.sizeof(_native.Options) == 8 * 4 + 8
 
 
 def test_errors_without_gpu_are_status_codes(lib):
