@@ -17,7 +17,7 @@ ERRORS = {
 }
 PREC_FP64, PREC_FP32 = 0, 1
 BATCH_FUSED, BATCH_PERGRAIN = 0, 1
-CONV_AUTO, CONV_STENCIL, CONV_FFT = 0, 1, 2
+CONV_AUTO, CONV_CIRCULANT = 0, 1
 
 c_int = ctypes.c_int32
 c_dbl = ctypes.c_double

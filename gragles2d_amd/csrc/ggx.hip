@@ -91,6 +91,7 @@ struct ggx_ctx {
     int lds_per_cu = 163840;
     int static_conv[2] = {0, 0};   // static LDS bytes of k_conv_circulant<double/float>
     int static_level[2] = {0, 0};  // static LDS bytes of k_level<double/float>
+    int static_dense[2] = {0, 0};  // static LDS bytes of k_conv_dense<double/float>
 
     // circulant kernels
     long long* kern_off = nullptr;
@@ -164,19 +165,24 @@ void circulant(int n, double dt, int R, std::vector<long double>& K) {
     }
 }
 
+// Offset of the table of box side n: sum of ktab_len(m) for m < n.
+long long kern_offset(int n) { return (long long)n * (n - 1) + 2LL * kKPad * n; }
+
 int ensure_kernels(ggx_ctx* ctx, int nmax) {
     if (nmax >= kMaxKernN) return fail(ctx, GGX_E_BOUNDS, "box side exceeds the circulant table");
     std::vector<long double> K;
     for (int n = 1; n <= nmax; ++n) {
         if (ctx->kern_ready[n]) continue;
         circulant(n, ctx->d.dt, ctx->d.real_domain_size, K);
-        long long o = (long long)n * (n - 1) / 2;
+        const long long o = kern_offset(n), len = ktab_len(n);
+        std::vector<long double> v((size_t)len);
+        for (long long t = 0; t < len; ++t) v[(size_t)t] = K[(size_t)(((t - kKPad) % n + n) % n)];
         if (ctx->esize == 8) {
-            std::vector<double> v(K.begin(), K.end());
-            HIPCHK(hipMemcpy((double*)ctx->kern + o, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+            std::vector<double> w(v.begin(), v.end());
+            HIPCHK(hipMemcpy((double*)ctx->kern + o, w.data(), w.size() * 8, hipMemcpyHostToDevice));
         } else {
-            std::vector<float> v(K.begin(), K.end());
-            HIPCHK(hipMemcpy((float*)ctx->kern + o, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+            std::vector<float> w(v.begin(), v.end());
+            HIPCHK(hipMemcpy((float*)ctx->kern + o, w.data(), w.size() * 4, hipMemcpyHostToDevice));
         }
         ctx->kern_ready[n] = 1;
     }
@@ -207,6 +213,7 @@ int raise_lds(ggx_ctx* ctx) {
     // allow the dynamic-LDS kernels to use up to `lim`
     const void* fns[] = {(const void*)k_conv_circulant<double>, (const void*)k_conv_circulant<float>,
                          (const void*)k_level<double, true>, (const void*)k_level<float, true>,
+                         (const void*)k_conv_dense<double>, (const void*)k_conv_dense<float>,
                          (const void*)k_level<double, false>, (const void*)k_level<float, false>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -217,8 +224,9 @@ int raise_lds(ggx_ctx* ctx) {
     }
     ctx->lds_max = lim;
     // static __shared__ bytes count against the same per-workgroup limit as the dynamic request
-    int* dst[] = {&ctx->static_conv[0], &ctx->static_conv[1], &ctx->static_level[0], &ctx->static_level[1]};
-    for (int k = 0; k < 4; ++k) {
+    int* dst[] = {&ctx->static_conv[0], &ctx->static_conv[1], &ctx->static_level[0], &ctx->static_level[1],
+                  &ctx->static_dense[0], &ctx->static_dense[1]};
+    for (int k = 0; k < 6; ++k) {
         hipFuncAttributes fa;
         HIPCHK(hipFuncGetAttributes(&fa, fns[k]));
         *dst[k] = (int)fa.sharedSizeBytes;
@@ -256,6 +264,31 @@ std::vector<hipEvent_t> take_events(ggx_ctx* ctx, int count) {
     return ev;
 }
 
+// Convolution of the grains of one work list whose boxes have side <= nb: the register-blocked
+// kernel when both of its tiles fit in LDS, else the strip kernel.
+template <class T>
+int launch_conv(ggx_ctx* ctx, const int* list, const int* count, int* cursor, int nb, int grid, const T* src,
+                T* dst) {
+    const int ti = sizeof(T) == 8 ? 0 : 1;
+    hipStream_t st = ctx->stream;
+    const long long dense = dense_lds<T>(nb);
+    if (dense + ctx->static_dense[ti] + 256 <= ctx->lds_max) {
+        if (grid <= 0) grid = ctx->cu_count * per_cu(ctx, dense + ctx->static_dense[ti]);
+        hipLaunchKernelGGL(k_conv_dense<T>, dim3(grid), dim3(256), (size_t)dense, st, list, count, cursor, ctx->box,
+                           ctx->off, src, dst, ctx->kern_off, (const T*)ctx->kern);
+        return GGX_OK;
+    }
+    const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;
+    long long kkn = (2LL * nb + 4 + 1) & ~1LL;
+    long long need = (kkn + 2LL * nb * nb) * (long long)sizeof(T);
+    long long lds = std::min<long long>(need, conv_max);
+    if ((kkn + 4LL * nb) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
+    if (grid <= 0) grid = ctx->cu_count * per_cu(ctx, lds + ctx->static_conv[ti]);
+    hipLaunchKernelGGL(k_conv_circulant<T>, dim3(grid), dim3(256), (size_t)lds, st, list, count, cursor, ctx->box,
+                       ctx->off, src, dst, ctx->kern_off, (const T*)ctx->kern, nb, (int)lds);
+    return GGX_OK;
+}
+
 template <class T>
 int launch_step(ggx_ctx* ctx) {
     hipStream_t st = ctx->stream;
@@ -267,7 +300,6 @@ int launch_step(ggx_ctx* ctx) {
     uint32_t* Iold = ctx->I[ctx->icur];
     uint32_t* Inew = ctx->I[ctx->icur ^ 1];
     const int ti = sizeof(T) == 8 ? 0 : 1;
-    const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;    // dynamic LDS caps
     const int level_max = ctx->lds_max - ctx->static_level[ti] - 256;
     const int cus = ctx->cu_count;
     Sched* S = ctx->sched;
@@ -289,17 +321,11 @@ int launch_step(ggx_ctx* ctx) {
     }
     // convolution, one launch per bucket
     for (int b = 0; b < kNumBuckets; ++b) {
-        int nb = kBucketEdges[b];
-        long long kkn = (2LL * nb + 4 + 1) & ~1LL;
-        long long need = (kkn + 2LL * nb * nb) * (long long)sizeof(T);
-        long long lds = std::min<long long>(need, conv_max);
-        if ((kkn + 4LL * nb) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
-        int grid = cus * per_cu(ctx, lds + ctx->static_conv[ti]);
-        if (!ctx->opt.persistent) grid = counts[b];
-        if (grid == 0) continue;
-        hipLaunchKernelGGL(k_conv_circulant<T>, dim3(grid), dim3(256), (size_t)lds, st, ctx->blists + (size_t)b * NS,
-                           &S->bucket_count[b], &S->bucket_cursor[b], ctx->box, ctx->off, (const T*)P, Tt,
-                           ctx->kern_off, (const T*)ctx->kern, nb, (int)lds);
+        int grid = ctx->opt.persistent ? 0 : counts[b];
+        if (!ctx->opt.persistent && grid == 0) continue;
+        int rc = launch_conv<T>(ctx, ctx->blists + (size_t)b * NS, &S->bucket_count[b], &S->bucket_cursor[b],
+                                kBucketEdges[b], grid, (const T*)P, Tt);
+        if (rc) return rc;
     }
     HIPCHK(hipEventRecord(ev[2], st));
     // fused comparison / level set / redistancing, one persistent launch per bucket
@@ -323,6 +349,7 @@ int launch_step(ggx_ctx* ctx) {
         A.O = O; A.G = G; A.loop = ctx->loop; A.analysis_timestep = ctx->s.analysis_timestep;
         A.nb = nb; A.lds_bytes = (int)lds;
         A.cursor = &S->bucket_cursor2[b];
+        A.gather_total = &S->gather_total; A.out_total = &S->out_total;
         int grid = cus * per_cu(ctx, lds + ctx->static_level[ti]);
         if (!ctx->opt.persistent) grid = counts[b];
         if (grid == 0) continue;
@@ -367,7 +394,6 @@ int launch_step_pergrain(ggx_ctx* ctx) {
     uint32_t* Inew = ctx->I[ctx->icur ^ 1];
     Sched* S = ctx->sched;
     const int ti = sizeof(T) == 8 ? 0 : 1;
-    const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;
     const int level_max = ctx->lds_max - ctx->static_level[ti] - 256;
     std::vector<hipEvent_t> ev = take_events(ctx, 5);
     // host view of the live grains and their boxes
@@ -393,13 +419,9 @@ int launch_step_pergrain(ggx_ctx* ctx) {
                            ctx->box, ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
     HIPCHK(hipEventRecord(ev[1], st));
     for (size_t i = 0; i < live.size(); ++i) {
-        int n = bx[live[i]].n;
-        long long kkn = (2LL * n + 4 + 1) & ~1LL;
-        long long lds = std::min<long long>((kkn + 2LL * n * n) * (long long)sizeof(T), conv_max);
-        if ((kkn + 4LL * n) * (long long)sizeof(T) > lds) return fail(ctx, GGX_E_BOUNDS, "box too large for LDS strips");
-        hipLaunchKernelGGL(k_conv_circulant<T>, dim3(1), dim3(256), (size_t)lds, st, ctx->pg_list + i, ctx->pg_one + i,
-                           ctx->pg_cursor + L + i, ctx->box, ctx->off, (const T*)P, Tt, ctx->kern_off, (const T*)ctx->kern, n,
-                           (int)lds);
+        int rc = launch_conv<T>(ctx, ctx->pg_list + i, ctx->pg_one + i, ctx->pg_cursor + L + i, bx[live[i]].n, 1,
+                                (const T*)P, Tt);
+        if (rc) return rc;
     }
     HIPCHK(hipEventRecord(ev[2], st));
     ContourOut O;
@@ -422,6 +444,7 @@ int launch_step_pergrain(ggx_ctx* ctx) {
         A.O = O; A.G = G; A.loop = ctx->loop; A.analysis_timestep = ctx->s.analysis_timestep;
         A.nb = n; A.lds_bytes = (int)lds;
         A.cursor = ctx->pg_cursor + 2 * L + i;
+        A.gather_total = &S->gather_total; A.out_total = &S->out_total;
         if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(1), dim3(256), (size_t)lds, st, A);
         else hipLaunchKernelGGL((k_level<T, false>), dim3(1), dim3(256), (size_t)lds, st, A);
     }
@@ -653,11 +676,11 @@ extern "C" int ggx_load_network(ggx_ctx* ctx, const ggx_network* net) {
         ctx->h_sched = (Sched*)hp;
         std::memset(ctx->h_sched, 0, sizeof(Sched));
     }
-    HIPCHK(hipMalloc(&p, (size_t)kMaxKernN * (kMaxKernN - 1) / 2 * es + 64)); ctx->kern = p;
+    HIPCHK(hipMalloc(&p, (size_t)kern_offset(kMaxKernN) * es + 64)); ctx->kern = p;
     ctx->kern_ready.assign(kMaxKernN, 0);
     {
         std::vector<long long> ko(kMaxKernN);
-        for (int n = 0; n < kMaxKernN; ++n) ko[n] = (long long)n * (n - 1) / 2;
+        for (int n = 0; n < kMaxKernN; ++n) ko[n] = kern_offset(n);
         HIPCHK(hipMemcpy(ctx->kern_off, ko.data(), ko.size() * 8, hipMemcpyHostToDevice));
     }
     if ((rc = ensure_kernels(ctx, std::min(kMaxKernN - 1, nmax + 32)))) return rc;
@@ -922,6 +945,11 @@ extern "C" int ggx_get_stats(ggx_ctx* ctx, ggx_stats* out) {
         ctx->ev_pending.clear();
         const Sched& H = *ctx->h_sched;
         ctx->st.grid_point_updates = (int64_t)H.updates_total;
+        const int64_t S = ctx->esize;
+        ctx->st.conv_bytes = 2 * S * (int64_t)H.updates_total;
+        // comparison: own phi~ + neighbour gathers; IDLocal written on the box; phi' on the new box
+        ctx->st.algorithmic_bytes = ctx->st.conv_bytes + S * (int64_t)(H.updates_total + H.gather_total + H.out_total) +
+                                    4 * (int64_t)H.updates_total;
         ctx->st.live_grains = H.all_count;
         ctx->st.max_box = H.max_n;
     }

@@ -62,6 +62,8 @@ struct Sched {
     unsigned long long tops[2];      // bump allocators: field arena, contour arena
     unsigned long long updates_total;  // sum of n^2 over all steps (never reset)
     unsigned long long steps_total;
+    unsigned long long gather_total;   // sum over steps and grains of the neighbour pixels compared
+    unsigned long long out_total;      // sum over steps and grains of the redistanced box areas
 };
 
 struct Buckets {
@@ -289,6 +291,12 @@ __global__ void __launch_bounds__(256) k_lists(const int* __restrict__ alist, co
 }
 
 // ------------------------------------------------------------------------------------------
+// Circulant tables: the table of box side n holds K_n[(t - kKPad) mod n] for
+// t in [0, 2n + 2 kKPad), so that every index a register block needs is in range without a
+// modulo.
+constexpr int kKPad = 8;
+__host__ __device__ inline long long ktab_len(int n) { return 2LL * n + 2 * kKPad; }
+
 // Convolution as an exact dense circulant (LSbox::convolutionGeneratorFFTW, box.cpp:622-695,
 // Gaussian mode).  The Gaussian symbol G(i,j) = g(i) g(j) is separable and even, so
 // IDFT(G . DFT(phi)) = K_n phi K_n with the symmetric circulant
@@ -407,9 +415,147 @@ __global__ void __launch_bounds__(256) k_conv_circulant(const int* __restrict__ 
         int n = box[g].n;
         long long o = off[g];
         const T* kg = kern + kern_off[n];
-        for (int t = threadIdx.x; t < 2 * n + S; t += blockDim.x) kk[t] = kg[t % n];
+        for (int t = threadIdx.x; t < 2 * n + S; t += blockDim.x) kk[t] = kg[kKPad + t % n];
         __syncthreads();
         conv_one<T>(n, src + o, dst + o, kk, A, avail);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// The same circulant convolution, register-blocked for boxes whose two tiles fit in LDS.
+// Row pass: lane = row y, each task a block of 8 outputs x0..x0+7 of the wave's rows,
+//   B[y][x] = sum_j A[y][j] K[x - j];
+// column pass: lane = column x, each task a block of 8 outputs y0..y0+7,
+//   D[y][x] = sum_i B[i][x] K[y - i].
+// The circulant coefficients of a block depend only on the task, so they are wave-uniform and
+// come through the scalar cache into SGPRs; LDS only feeds the field values (one 8-byte read per
+// 8 FMAs).  The tiles are zero-padded to n8 = 8 ceil(n/8) rows and columns and their pitch
+// (n8 + 2 doubles, n8 + 4 floats) keeps the per-lane 16-byte row reads free of bank conflicts.
+// ------------------------------------------------------------------------------------------
+template <class T>
+__host__ __device__ inline int dense_pitch(int n8) { return n8 + (sizeof(T) == 8 ? 2 : 4); }
+
+template <class T>
+__host__ __device__ inline long long dense_lds(int nb) {
+    const int n8 = (nb + 7) & ~7;
+    return 2LL * n8 * dense_pitch<T>(n8) * (long long)sizeof(T);
+}
+
+template <class T>
+using ctab_ptr = const __attribute__((address_space(4))) T*;
+
+template <class T>
+__device__ __forceinline__ void load8(const T* p, T* a) {
+    if constexpr (sizeof(T) == 8) {
+        const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { double2 v = q[k]; a[2 * k] = v.x; a[2 * k + 1] = v.y; }
+    } else {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) { float4 v = q[k]; a[4 * k] = v.x; a[4 * k + 1] = v.y; a[4 * k + 2] = v.z; a[4 * k + 3] = v.w; }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void store8(T* p, const T* a) {
+    if constexpr (sizeof(T) == 8) {
+        double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = make_double2(a[2 * k], a[2 * k + 1]);
+    } else {
+        float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) q[k] = make_float4(a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]);
+    }
+}
+
+// acc[q] += sum_u v[u] * W[q - u + 7], W = 15 consecutive (wave-uniform) table entries
+template <class T>
+__device__ __forceinline__ void block8(T* acc, const T* v, ctab_ptr<T> w) {
+    T W[15];
+#pragma unroll
+    for (int t = 0; t < 15; ++t) W[t] = w[t];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += v[u] * W[q - u + 7];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list, const int* __restrict__ count,
+                                                    int* __restrict__ cursor, const Box* __restrict__ box,
+                                                    const long long* __restrict__ off, const T* __restrict__ src,
+                                                    T* __restrict__ dst, const long long* __restrict__ kern_off,
+                                                    const T* __restrict__ kern) {
+    HIP_DYNAMIC_SHARED(unsigned char, smem)
+    __shared__ int s_slot;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nwaves = (int)(blockDim.x >> 6);
+    while (true) {
+        const int item = __builtin_amdgcn_readfirstlane(next_item(cursor, &s_slot));
+        if (item >= *count) break;
+        const int g = list[item];
+        const int n = box[g].n;
+        const long long o = off[g];
+        const int n8 = (n + 7) & ~7;
+        const int pa = dense_pitch<T>(n8);
+        T* A = (T*)smem;
+        T* B = A + (long long)n8 * pa;
+        const ctab_ptr<T> kt = (ctab_ptr<T>)(kern + kern_off[n]);
+        const T* s = src + o;
+        // A = phi, zero-padded to n8 x n8
+        for (int p = threadIdx.x; p < n8 * n8; p += blockDim.x) {
+            const int y = p / n8, x = p - y * n8;
+            A[y * pa + x] = (y < n && x < n) ? s[y * n + x] : T(0);
+        }
+        __syncthreads();
+        // row pass
+        const int xb = n8 >> 3, rg = (n8 + 63) >> 6;
+        for (int task = wave; task < rg * xb; task += nwaves) {
+            const int r64 = task / xb, bx = task - r64 * xb;
+            const int y = r64 * 64 + lane, x0 = bx * 8;
+            T acc[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] = T(0);
+            const T* arow = A + (long long)y * pa;
+            const bool on = y < n8;
+            for (int j0 = 0; j0 < n8; j0 += 8) {
+                T v[8];
+                if (on) load8(arow + j0, v);
+                else {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = T(0);
+                }
+                block8(acc, v, kt + (x0 - j0 + n + kKPad - 7));
+            }
+            if (on) store8(B + (long long)y * pa + x0, acc);
+        }
+        __syncthreads();
+        // column pass
+        const int yb = n8 >> 3, cg = (n + 63) >> 6;
+        T* d = dst + o;
+        for (int task = wave; task < cg * yb; task += nwaves) {
+            const int c64 = task / yb, by = task - c64 * yb;
+            const int x = c64 * 64 + lane, y0 = by * 8;
+            const bool on = x < n;
+            T acc[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) acc[r] = T(0);
+            for (int i0 = 0; i0 < n8; i0 += 8) {
+                T v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = on ? B[(long long)(i0 + u) * pa + x] : T(0);
+                block8(acc, v, kt + (y0 - i0 + n + kKPad - 7));
+            }
+            if (on) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if (y0 + r < n) d[(long long)(y0 + r) * n + x] = acc[r];
+            }
+        }
+        __syncthreads();
     }
 }
 

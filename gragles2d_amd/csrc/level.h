@@ -38,6 +38,8 @@ struct LevelArgs {
     GridConst G;
     int loop;
     int analysis_timestep;
+    unsigned long long* gather_total;  // running sum of neighbour pixels gathered by the comparison
+    unsigned long long* out_total;     // running sum of the new boxes' areas
     int nb;                    // largest box side of the bucket (sizes the LDS regions)
     int lds_bytes;
 };
@@ -60,12 +62,12 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
 
 // Interpolated point of a recorded move (marchingSquares.cpp:204-265).
 template <class T>
-__device__ __forceinline__ double2 move_point(const T* C, int n, int x0, int y0, uint32_t mv) {
+__device__ __forceinline__ double2 move_point(const T* C, int n, int cp, int x0, int y0, uint32_t mv) {
     int dir = (int)(mv & 3u);
     int left = (int)((mv >> 2) & 0x3fffu) + x0 - 1;
     int top = (int)((mv >> 16) & 0x3fffu) + y0 - 1;
     int right = left + 1, bottom = top + 1;
-    auto val = [&](int row, int col) { return (double)C[(row - y0) * n + (col - x0)]; };
+    auto val = [&](int row, int col) { return (double)C[(row - y0) * cp + (col - x0)]; };
     double2 r;
     double delta;
     switch (dir) {
@@ -93,6 +95,125 @@ __device__ __forceinline__ double2 move_point(const T* C, int n, int x0, int y0,
     return r;
 }
 
+// The square walk of MarchingSquaresAlgorithm::generateContour (marchingSquares.cpp:44-176)
+// from the first inside pixel (sx, sy): inside(row, col) is the inside test (value >= 0 within
+// the box), emit(k, move) records the k-th point-generating move packed as direction (2 bits),
+// left - x0 + 1 (14 bits) and top - y0 + 1 (16 bits); the two corners shared with the next
+// square are reused.  Returns 0, kErrContourCap or kErrTrace; *nmoves gets the move count.
+template <class Inside, class Emit>
+__device__ __forceinline__ int walk_squares(int sx, int sy, int x0, int y0, int n, int ccap, Inside inside, Emit emit,
+                                            int* nmoves) {
+    int left = sx - 1, top = sy - 1;
+    int next = D_NONE, prev = D_NONE;
+    const long long itcap = 8LL * (n + 2) * (n + 2) + 16;
+    long long it = 0;
+    int nm = 0, st = 0;
+    bool tl = inside(top, left), tr = inside(top, left + 1);
+    bool bl = inside(top + 1, left), br = inside(top + 1, left + 1);
+    do {
+        const int mask = (tl ? MS_TL : 0) | (tr ? MS_TR : 0) | (bl ? MS_BL : 0) | (br ? MS_BR : 0);
+        bool gen = true;
+        switch (mask) {
+            case MS_TL: next = D_UP; break;
+            case MS_TR: next = D_RIGHT; break;
+            case MS_TL | MS_TR: next = D_RIGHT; break;
+            case MS_BL: next = D_LEFT; break;
+            case MS_BL | MS_TL: next = D_UP; break;
+            case MS_BL | MS_TR: next = (prev == D_UP) ? D_RIGHT : D_LEFT; gen = false; break;
+            case MS_BL | MS_TR | MS_TL: next = D_RIGHT; break;
+            case MS_BR: next = D_DOWN; break;
+            case MS_BR | MS_TL: next = (prev == D_RIGHT) ? D_DOWN : D_UP; gen = false; break;
+            case MS_BR | MS_TR: next = D_DOWN; break;
+            case MS_BR | MS_TR | MS_TL: next = D_DOWN; break;
+            case MS_BR | MS_BL: next = D_LEFT; break;
+            case MS_BR | MS_BL | MS_TL: next = D_UP; break;
+            case MS_BR | MS_BL | MS_TR: next = D_LEFT; break;
+            default: gen = false; break;
+        }
+        if (gen) {
+            if (nm >= ccap) { st = kErrContourCap; break; }
+            emit(nm++, (uint32_t)next | ((uint32_t)(left - x0 + 1) << 2) | ((uint32_t)(top - y0 + 1) << 16));
+        }
+        switch (next) {
+            case D_UP: top--; bl = tl; br = tr; tl = inside(top, left); tr = inside(top, left + 1); break;
+            case D_DOWN: top++; tl = bl; tr = br; bl = inside(top + 1, left); br = inside(top + 1, left + 1); break;
+            case D_LEFT: left--; tr = tl; br = bl; tl = inside(top, left); bl = inside(top + 1, left); break;
+            case D_RIGHT: left++; tl = tr; bl = br; tr = inside(top, left + 1); br = inside(top + 1, left + 1); break;
+            default: break;
+        }
+        prev = next;
+        if (++it > itcap) { st = kErrTrace; break; }
+    } while (left != sx - 1 || top + 1 != sy);
+    *nmoves = nm;
+    return st;
+}
+
+// Inside bits (value >= 0) of a box up to 128 x 128 in the VGPRs of one wave: lane l holds
+// rows l and l + 64, each as two 64-column words, built with one ballot per row and word.
+// The walk then runs wave-uniformly in scalar registers, fetching rows with readlane.
+struct WaveMask {
+    unsigned long long m00, m01, m10, m11;  // [row >= 64][column >= 64]
+};
+
+template <class T>
+__device__ __forceinline__ WaveMask wave_masks(const T* C, int n, int cp) {
+    const int lane = threadIdx.x & 63;
+    WaveMask M{0ull, 0ull, 0ull, 0ull};
+#pragma unroll 4
+    for (int r = 0; r < n; ++r) {
+        const T* row = C + (long long)r * cp;
+        const unsigned long long w0 = __ballot(lane < n && row[lane] >= T(0));
+        const unsigned long long w1 = n > 64 ? __ballot(64 + lane < n && row[64 + lane] >= T(0)) : 0ull;
+        if (lane == (r & 63)) {
+            if (r < 64) { M.m00 = w0; M.m01 = w1; }
+            else { M.m10 = w0; M.m11 = w1; }
+        }
+    }
+    return M;
+}
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int src) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Wave 0 of the workgroup: start pixel (first inside pixel in row-major order) and walk.
+// Writes s_int[0] (start index or INT_MAX when nothing is inside), s_int[1] (moves) and, on
+// failure, s_int[3].
+__device__ __forceinline__ void wave_trace(const WaveMask& M, int n, int x0, int y0, int ccap, uint32_t* moves,
+                                           int* s_int) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long any0 = __ballot((M.m00 | M.m01) != 0ull);
+    const unsigned long long any1 = __ballot((M.m10 | M.m11) != 0ull);
+    int r;
+    if (any0) r = __builtin_ctzll(any0);
+    else if (any1) r = 64 + __builtin_ctzll(any1);
+    else {
+        if (lane == 0) s_int[0] = 0x7fffffff;
+        return;
+    }
+    const unsigned long long a = readlane64(r < 64 ? M.m00 : M.m10, r & 63);
+    const unsigned long long b = readlane64(r < 64 ? M.m01 : M.m11, r & 63);
+    const int c = a ? __builtin_ctzll(a) : 64 + __builtin_ctzll(b);
+    auto inside = [&](int row, int col) -> bool {
+        const int rr = row - y0, q = col - x0;
+        if ((unsigned)rr >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
+        const unsigned long long v = rr < 64 ? (q < 64 ? M.m00 : M.m01) : (q < 64 ? M.m10 : M.m11);
+        return (readlane64(v, rr & 63) >> (q & 63)) & 1ull;
+    };
+    auto emit = [&](int k, uint32_t mv) {
+        if (lane == 0) moves[k] = mv;
+    };
+    int nm = 0;
+    const int st = walk_squares(x0 + c, y0 + r, x0, y0, n, ccap, inside, emit, &nm);
+    if (lane == 0) {
+        s_int[0] = r * n + c;
+        s_int[1] = nm;
+        if (st) s_int[3] = st;
+    }
+}
+
 // LDS layout of the fused kernel for boxes up to side nb.  Full layout: C tile, O tile (which
 // also hosts the per-point work arrays until redistancing), moves, scan scratch.  Small
 // layout (boxes too large for LDS): C and O stay in global memory, only the work arrays,
@@ -107,7 +228,7 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     long long ccap = contour_cap(nb);
     long long no = nb + 8;
     long long work = (ccap * 20 + 15) & ~15LL;
-    L.ctile = full ? (((long long)nb * nb * (long long)sizeof(T) + 15) & ~15LL) : 0;
+    L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
     L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
     if (work > L.otile) L.otile = work;
     L.moves = (ccap * 4 + 15) & ~15LL;
@@ -117,11 +238,12 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
 }
 
 // LSbox::executeRedistancing (box.cpp:1259-1399) of one grain: Ob is the output tile (LDS with
-// an odd pitch, or the global slot itself), C the compared field on the old box b, bo the new
-// box.  Lanes own columns (y sweeps) or rows (x sweeps); the value carried along a sweep stays
-// in a register.
+// an odd pitch, or the global slot itself), C the compared field on the old box b (row pitch
+// cp), bo the new box.  Lanes own columns (y sweeps) or rows (x sweeps); the value carried
+// along a sweep stays in a register and the next line's inputs are loaded one step ahead (a
+// sweep step only writes its own two lines, so the early loads see final values).
 template <class T>
-__device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, Box b, Box bo, const GridConst& G,
+__device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, int cp, Box b, Box bo, const GridConst& G,
                                            T* gout) {
     const int n = b.n, no = bo.n;
     const double h = G.h;
@@ -132,7 +254,7 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, Box b, 
     const int ixmax = ixmx < oxmax ? ixmx : oxmax;
     const int iymax = iymx < oymax ? iymx : oymax;
 #define OV(i, j) Ob[((i) - oymin) * pitch + ((j) - oxmin)]
-#define CI(i, j) C[((i) - iymn) * n + ((j) - ixmn)]
+#define CI(i, j) C[((i) - iymn) * cp + ((j) - ixmn)]
     for (int p = threadIdx.x; p < no * no; p += blockDim.x) {
         int yy = p / no, xx = p - yy * no;
         Ob[yy * pitch + xx] = T(-1.0);
@@ -141,74 +263,98 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, Box b, 
     // y forward (box.cpp:1291-1319)
     for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
         if (iymin >= oymax - 1) break;
+        const bool colin = j < ixmax;
         T cur = OV(iymin, j);
+        T o1 = OV(iymin + 1, j);
+        double a = (colin && iymin < iymax) ? (double)CI(iymin, j) : 0.0;
+        double a1 = (colin && iymin + 1 < iymax) ? (double)CI(iymin + 1, j) : 0.0;
         for (int i = iymin; i < oymax - 1; i++) {
-            T nxt = OV(i + 1, j);
+            const T o2 = (i + 2 < oymax) ? OV(i + 2, j) : T(0);
+            const double a2 = (colin && i + 2 < iymax) ? (double)CI(i + 2, j) : 0.0;
+            T nxt = o1;
             double cand;
-            if (j < ixmax && i < iymax - 1) {
-                double a = (double)CI(i, j), an = (double)CI(i + 1, j);
-                if (a * an <= 0.0) {
-                    double slope = (an - a) / h;
+            if (colin && i < iymax - 1) {
+                if (a * a1 <= 0.0) {
+                    double slope = (a1 - a) / h;
                     double d0 = -a / slope;
                     if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
                 }
-                cand = (double)cur + (sgn_of(CI(i + 1, j)) * h);
+                cand = (double)cur + (sgn_of(a1) * h);
             } else {
                 cand = (double)cur + (sgn_of(nxt) * h);
             }
             if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i + 1, j) = nxt; }
             cur = nxt;
+            a = a1; a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
     // y backward (1321-1338)
     for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
         if (iymax - 1 <= oymin) break;
+        const bool colin = j < ixmax;
         T cur = OV(iymax - 1, j);
+        T o1 = OV(iymax - 2, j);
+        double a1 = (colin && iymax - 2 >= iymin) ? (double)CI(iymax - 2, j) : 0.0;
         for (int i = iymax - 1; i > oymin; i--) {
-            T nxt = OV(i - 1, j);
+            const T o2 = (i - 2 >= oymin) ? OV(i - 2, j) : T(0);
+            const double a2 = (colin && i - 2 >= iymin) ? (double)CI(i - 2, j) : 0.0;
+            T nxt = o1;
             double cand;
-            if (j < ixmax && i > iymin) cand = (double)cur + (sgn_of(CI(i - 1, j)) * h);
+            if (colin && i > iymin) cand = (double)cur + (sgn_of(a1) * h);
             else cand = (double)cur + (sgn_of(nxt) * h);
             if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i - 1, j) = nxt; }
             cur = nxt;
+            a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
     // x forward (1339-1367)
     for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
         if (ixmin >= oxmax - 1) break;
+        const bool rowin = i < iymax;
         T cur = OV(i, ixmin);
+        T o1 = OV(i, ixmin + 1);
+        double a = (rowin && ixmin < ixmax) ? (double)CI(i, ixmin) : 0.0;
+        double a1 = (rowin && ixmin + 1 < ixmax) ? (double)CI(i, ixmin + 1) : 0.0;
         for (int j = ixmin; j < oxmax - 1; j++) {
-            T nxt = OV(i, j + 1);
+            const T o2 = (j + 2 < oxmax) ? OV(i, j + 2) : T(0);
+            const double a2 = (rowin && j + 2 < ixmax) ? (double)CI(i, j + 2) : 0.0;
+            T nxt = o1;
             double cand;
-            if (j < ixmax - 1 && i < iymax) {
-                double a = (double)CI(i, j), an = (double)CI(i, j + 1);
-                if (a * an <= 0.0) {
-                    double slope = (an - a) / h;
+            if (j < ixmax - 1 && rowin) {
+                if (a * a1 <= 0.0) {
+                    double slope = (a1 - a) / h;
                     double d0 = -a / slope;
                     if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
                 }
-                cand = (double)cur + (sgn_of(CI(i, j + 1)) * h);
+                cand = (double)cur + (sgn_of(a1) * h);
             } else {
                 cand = (double)cur + (sgn_of(nxt) * h);
             }
             if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j + 1) = nxt; }
             cur = nxt;
+            a = a1; a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
     // x backward (1369-1388)
     for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
         if (ixmax - 1 <= oxmin) break;
+        const bool rowin = i < iymax;
         T cur = OV(i, ixmax - 1);
+        T o1 = OV(i, ixmax - 2);
+        double a1 = (rowin && ixmax - 2 >= ixmin) ? (double)CI(i, ixmax - 2) : 0.0;
         for (int j = ixmax - 1; j > oxmin; j--) {
-            T nxt = OV(i, j - 1);
+            const T o2 = (j - 2 >= oxmin) ? OV(i, j - 2) : T(0);
+            const double a2 = (rowin && j - 2 >= ixmin) ? (double)CI(i, j - 2) : 0.0;
+            T nxt = o1;
             double cand;
-            if (j > ixmin && i < iymax) cand = (double)cur + (sgn_of(CI(i, j - 1)) * h);
+            if (j > ixmin && rowin) cand = (double)cur + (sgn_of(a1) * h);
             else cand = (double)cur + sgn_of(nxt) * h;
             if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j - 1) = nxt; }
             cur = nxt;
+            a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
@@ -242,12 +388,15 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
     int* s_scan = (int*)((unsigned char*)moves + LY.moves);
 
     __shared__ int s_bb[4];
-    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len, 7 stop
+    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len, 7 neighbours
     __shared__ double s_dbl[8];   // 0 volume sum, 1 minx, 2 maxx, 3 miny, 4 maxy
     __shared__ int s_id[kCmpCap], s_x0[kCmpCap], s_y0[kCmpCap], s_n[kCmpCap];
     __shared__ long long s_off[kCmpCap];
     __shared__ long long s_coff, s_onext;
     __shared__ Box s_nbx;
+    __shared__ unsigned s_hits;
+    __shared__ int s_nbid[kNbCap];
+    __shared__ double s_nblen[kNbCap];
 
     __shared__ int s_slot;
     while (true) {
@@ -259,8 +408,9 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
         const long long o = A.off[g];
         const long long nn = (long long)n * n;
         T* C;
-        if constexpr (FULL) C = Cl;
-        else C = A.cglobal + o;
+        int cp;  // row pitch of C: odd in LDS (conflict-free column walks), n in the global slot
+        if constexpr (FULL) { C = Cl; cp = n | 1; }
+        else { C = A.cglobal + o; cp = n; }
 
         // ---------------- comparison + set-comparison ----------------
         const int cnt = A.cmp_cnt[g];
@@ -274,6 +424,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
         if (threadIdx.x == 0) {
             s_bb[0] = b.y0 + n; s_bb[1] = b.y0; s_bb[2] = b.x0 + n; s_bb[3] = b.x0;
             s_int[3] = 0; s_int[4] = 1; s_int[5] = 1;
+            s_hits = 0;
             unsigned long long co = atomicAdd(&A.O.tops[1], (unsigned long long)ccap);
             s_coff = (long long)co;
             if (co + (unsigned long long)ccap > (unsigned long long)A.O.contour_cap) s_int[3] = kErrContourCap;
@@ -286,44 +437,74 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const bool bnd = !(b.x0 > lo && b.x0 + n < hi && b.y0 > lo && b.y0 + n < hi);
             const double band = 0.7 * G.delta;
             int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
-            for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
-                int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
-                int i = b.y0 + yy, j = b.x0 + xx;
-                T mx = T(-1.0);
-                int id = -1;
+            unsigned hits = 0;
+            // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
+            // chunk, so the gathers of one neighbour are independent loads issued together;
+            // per pixel the neighbours are still met in list order (strict >, first wins).
+            constexpr int kPx = 8;
+            for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
+                int yy[kPx], xx[kPx], id[kPx];
+                T mx[kPx];
+#pragma unroll
+                for (int t = 0; t < kPx; ++t) {
+                    const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
+                    yy[t] = p < nn ? (int)(p / n) : n;  // row n lies outside every overlap
+                    xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
+                    mx[t] = T(-1.0);
+                    id[t] = -1;
+                }
+#pragma unroll 2
                 for (int k = 0; k < cnt; ++k) {
-                    int ly = i - s_y0[k], lx = j - s_x0[k];
-                    int nk = s_n[k];
-                    if ((unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk) {
-                        T v = A.tilde[s_off[k] + (long long)ly * nk + lx];
-                        if (v > mx) { mx = v; id = s_id[k]; }
+                    const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
+                    const T* nsrc = A.tilde + s_off[k];
+                    T v[kPx];
+                    bool in[kPx];
+#pragma unroll
+                    for (int t = 0; t < kPx; ++t) {
+                        const int ly = yy[t] - dy, lx = xx[t] - dx;
+                        in[t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
+                        v[t] = in[t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                    }
+#pragma unroll
+                    for (int t = 0; t < kPx; ++t) {
+                        hits += in[t] ? 1u : 0u;
+                        if (in[t] && v[t] > mx[t]) { mx[t] = v[t]; id[t] = s_id[k]; }
                     }
                 }
-                if (bnd) {
-                    int gb = G.gb, m = G.m;
-                    int dXMin = -(j - gb), dYMin = -(i - gb);
-                    int dXMax = (j - (m - gb)), dYMax = (i - (m - gb));
-                    int dX = (abs(dXMin) < abs(dXMax)) ? dXMin : dXMax;
-                    int dY = (abs(dYMin) < abs(dYMax)) ? dYMin : dYMax;
-                    double dist;
-                    if (dX > 0 && dY > 0) dist = sqrt((double)dX * dX + dY * dY);
-                    else if (dX < 0 && dY > 0) dist = dY;
-                    else if (dX > 0 && dY < 0) dist = dX;
-                    else if (dX < 0 && dY < 0) dist = max(dX, dY);
-                    else if (dX == 0) dist = (dY > 0) ? (double)dY : 0.0;
-                    else dist = (dX > 0) ? (double)dX : 0.0;
-                    if ((T)(dist * G.h) > mx) { mx = (T)(dist * G.h); id = 0; }
-                }
-                uint32_t idv = (id >= 0) ? (uint32_t)id : ((p < np) ? A.idl_old[op + p] : 0u);
-                A.idl_new[o + p] = idv;
-                T t = A.tilde[o + p];
-                T c = (fabs((double)t) < band) ? (T)(0.5 * ((double)t - (double)mx)) : t;
-                C[p] = c;
-                if (c >= T(0)) {
-                    ymin_l = min(ymin_l, i); ymax_l = max(ymax_l, i);
-                    xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
+#pragma unroll
+                for (int t = 0; t < kPx; ++t) {
+                    const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
+                    if (p >= nn) continue;
+                    const int i = b.y0 + yy[t], j = b.x0 + xx[t];
+                    T m = mx[t];
+                    int idt = id[t];
+                    if (bnd) {
+                        int gb = G.gb, m2 = G.m;
+                        int dXMin = -(j - gb), dYMin = -(i - gb);
+                        int dXMax = (j - (m2 - gb)), dYMax = (i - (m2 - gb));
+                        int dX = (abs(dXMin) < abs(dXMax)) ? dXMin : dXMax;
+                        int dY = (abs(dYMin) < abs(dYMax)) ? dYMin : dYMax;
+                        double dist;
+                        if (dX > 0 && dY > 0) dist = sqrt((double)dX * dX + dY * dY);
+                        else if (dX < 0 && dY > 0) dist = dY;
+                        else if (dX > 0 && dY < 0) dist = dX;
+                        else if (dX < 0 && dY < 0) dist = max(dX, dY);
+                        else if (dX == 0) dist = (dY > 0) ? (double)dY : 0.0;
+                        else dist = (dX > 0) ? (double)dX : 0.0;
+                        if ((T)(dist * G.h) > m) { m = (T)(dist * G.h); idt = 0; }
+                    }
+                    uint32_t idv = (idt >= 0) ? (uint32_t)idt : ((p < np) ? A.idl_old[op + p] : 0u);
+                    A.idl_new[o + p] = idv;
+                    T tv = A.tilde[o + p];
+                    T c = (fabs((double)tv) < band) ? (T)(0.5 * ((double)tv - (double)m)) : tv;
+                    C[yy[t] * cp + xx[t]] = c;
+                    if (c >= T(0)) {
+                        ymin_l = min(ymin_l, i); ymax_l = max(ymax_l, i);
+                        xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
+                    }
                 }
             }
+            if (hits) atomicAdd(&s_hits, hits);
             if (ymin_l != 0x7fffffff) {
                 atomicMin(&s_bb[0], ymin_l); atomicMax(&s_bb[1], ymax_l);
                 atomicMin(&s_bb[2], xmin_l); atomicMax(&s_bb[3], xmax_l);
@@ -338,6 +519,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
         if (threadIdx.x == 0) {  // this step's IDLocal is the stale source of the next step
             A.n_prev[g] = n;
             A.off_prev[g] = o;
+            atomicAdd(A.gather_total, (unsigned long long)s_hits);
         }
         if (s_int[3]) {
             if (threadIdx.x == 0) A.O.err[g] = s_int[3];
@@ -351,70 +533,35 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
         // ---------------- contour with motion check and retries ----------------
         int retries = 0;
         while (true) {
-            if (threadIdx.x == 0) s_int[0] = 0x7fffffff;
-            __syncthreads();
-            {
+            if (n <= 128) {
+                if (threadIdx.x < 64) wave_trace(wave_masks(C, n, cp), n, b.x0, b.y0, ccap, moves, s_int);
+                __syncthreads();
+            } else {
+                if (threadIdx.x == 0) s_int[0] = 0x7fffffff;
+                __syncthreads();
                 int loc = 0x7fffffff;
                 for (long long p = threadIdx.x; p < nn; p += blockDim.x)
-                    if (C[p] >= T(0)) { loc = (int)p; break; }
+                    if (C[(p / n) * cp + p % n] >= T(0)) { loc = (int)p; break; }
                 if (loc != 0x7fffffff) atomicMin(&s_int[0], loc);
+                __syncthreads();
+                if (threadIdx.x == 0 && s_int[0] != 0x7fffffff) {  // lane 0 walks the squares
+                    auto inside = [&](int row, int col) -> bool {
+                        int r = row - b.y0, q = col - b.x0;
+                        if ((unsigned)r >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
+                        return C[r * cp + q] >= T(0);
+                    };
+                    auto emit = [&](int k, uint32_t mv) { moves[k] = mv; };
+                    int nm = 0;
+                    int st = walk_squares(b.x0 + s_int[0] % n, b.y0 + s_int[0] / n, b.x0, b.y0, n, ccap, inside, emit, &nm);
+                    s_int[1] = nm;
+                    if (st) s_int[3] = st;
+                }
             }
             __syncthreads();
             if (s_int[0] == 0x7fffffff) {  // no inside point: the grain vanished
                 if (threadIdx.x == 0) { s_int[5] = 0; s_int[6] = 0; }
                 __syncthreads();
                 break;
-            }
-            // lane 0 walks the squares, recording point-generating moves
-            if (threadIdx.x == 0) {
-                const int sx = b.x0 + s_int[0] % n, sy = b.y0 + s_int[0] / n;
-                auto inside = [&](int row, int col) -> bool {
-                    int r = row - b.y0, q = col - b.x0;
-                    if ((unsigned)r >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
-                    return C[r * n + q] >= T(0);
-                };
-                int left = sx - 1, top = sy - 1;
-                int next = D_NONE, prev = D_NONE;
-                long long itcap = 8LL * (n + 2) * (n + 2) + 16, it = 0;
-                int nm = 0, st = 0;
-                bool tl = inside(top, left), tr = inside(top, left + 1);
-                bool bl = inside(top + 1, left), br = inside(top + 1, left + 1);
-                do {
-                    int mask = (tl ? MS_TL : 0) | (tr ? MS_TR : 0) | (bl ? MS_BL : 0) | (br ? MS_BR : 0);
-                    bool gen = true;
-                    switch (mask) {
-                        case MS_TL: next = D_UP; break;
-                        case MS_TR: next = D_RIGHT; break;
-                        case MS_TL | MS_TR: next = D_RIGHT; break;
-                        case MS_BL: next = D_LEFT; break;
-                        case MS_BL | MS_TL: next = D_UP; break;
-                        case MS_BL | MS_TR: next = (prev == D_UP) ? D_RIGHT : D_LEFT; gen = false; break;
-                        case MS_BL | MS_TR | MS_TL: next = D_RIGHT; break;
-                        case MS_BR: next = D_DOWN; break;
-                        case MS_BR | MS_TL: next = (prev == D_RIGHT) ? D_DOWN : D_UP; gen = false; break;
-                        case MS_BR | MS_TR: next = D_DOWN; break;
-                        case MS_BR | MS_TR | MS_TL: next = D_DOWN; break;
-                        case MS_BR | MS_BL: next = D_LEFT; break;
-                        case MS_BR | MS_BL | MS_TL: next = D_UP; break;
-                        case MS_BR | MS_BL | MS_TR: next = D_LEFT; break;
-                        default: gen = false; break;
-                    }
-                    if (gen) {
-                        if (nm >= ccap) { st = kErrContourCap; break; }
-                        moves[nm++] = (uint32_t)next | ((uint32_t)(left - b.x0 + 1) << 2) | ((uint32_t)(top - b.y0 + 1) << 16);
-                    }
-                    switch (next) {  // move and reuse the two shared corners
-                        case D_UP: top--; bl = tl; br = tr; tl = inside(top, left); tr = inside(top, left + 1); break;
-                        case D_DOWN: top++; tl = bl; tr = br; bl = inside(top + 1, left); br = inside(top + 1, left + 1); break;
-                        case D_LEFT: left--; tr = tl; br = bl; tl = inside(top, left); bl = inside(top + 1, left); break;
-                        case D_RIGHT: left++; tl = tr; bl = br; tr = inside(top, left + 1); br = inside(top + 1, left + 1); break;
-                        default: break;
-                    }
-                    prev = next;
-                    if (++it > itcap) { st = kErrTrace; break; }
-                } while (left != sx - 1 || top + 1 != sy);
-                s_int[1] = nm;
-                if (st) s_int[3] = st;
             }
             __syncthreads();
             if (s_int[3]) {
@@ -429,10 +576,10 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const int k0 = threadIdx.x * per, k1 = min(nm, k0 + per);
             int keep_cnt = 0;
             for (int k = k0; k < k1; ++k) {
-                double2 p = move_point(C, n, b.x0, b.y0, moves[k]);
+                double2 p = move_point(C, n, cp, b.x0, b.y0, moves[k]);
                 bool keep = true;
                 if (k > 0) {
-                    double2 q = move_point(C, n, b.x0, b.y0, moves[k - 1]);
+                    double2 q = move_point(C, n, cp, b.x0, b.y0, moves[k - 1]);
                     keep = (p.x != q.x || p.y != q.y);
                 }
                 keep_cnt += keep ? 1 : 0;
@@ -440,10 +587,10 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             int base = block_exclusive_scan(keep_cnt, s_scan);
             int w = base;
             for (int k = k0; k < k1; ++k) {
-                double2 p = move_point(C, n, b.x0, b.y0, moves[k]);
+                double2 p = move_point(C, n, cp, b.x0, b.y0, moves[k]);
                 bool keep = true;
                 if (k > 0) {
-                    double2 q = move_point(C, n, b.x0, b.y0, moves[k - 1]);
+                    double2 q = move_point(C, n, cp, b.x0, b.y0, moves[k - 1]);
                     keep = (p.x != q.x || p.y != q.y);
                 }
                 if (keep) {
@@ -485,6 +632,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 double vs = 0;
+#pragma unroll 8
                 for (int i = 0; i < len - 1; ++i) vs += s_term[i];
                 s_dbl[0] = vs;
                 double newVolume = fabs(vs) * G.h * G.h * 0.5;
@@ -525,7 +673,8 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                     int ii = i0 + p / jw, jj = j0 + p % jw;
                     long long k = flat_index(ii, jj, b.x0, b.y0, n, nn);
                     if (k < 0) { bad = 1; continue; }
-                    if (C[k] >= T(0)) C[k] = (T)(-G.h);
+                    const long long kc = (k / n) * cp + k % n;
+                    if (C[kc] >= T(0)) C[kc] = (T)(-G.h);
                 }
                 if (bad) atomicOr(&s_int[3], kErrClearBounds);
                 __threadfence_block();
@@ -581,9 +730,11 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             if ((long long)(fo + need) > A.O.field_cap) A.O.err[g] = kErrArena;
             A.O.box_next[g] = nbx;
             A.O.off_next[g] = (long long)fo;
+            atomicAdd(A.out_total, (unsigned long long)nbx.n * (unsigned long long)nbx.n);
             s_nbx = nbx;
             s_onext = (long long)fo;
             double per = 0;  // computePerimeter (grainBoundary.cpp:224-231)
+#pragma unroll 8
             for (int i = 0; i < len - 1; ++i) per += s_len[i] * G.h;
             A.O.perimeter[g] = per;
         }
@@ -597,15 +748,15 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                 bool ok = true;
                 long long k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
                 if (k < 0) ok = false;
-                else if (C[k] > T(0)) {
+                else if (C[(k / n) * cp + k % n] > T(0)) {
                     pxGrid = (int)(p.x + 1);
                     k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
                     if (k < 0) ok = false;
-                    else if (C[k] > T(0)) {
+                    else if (C[(k / n) * cp + k % n] > T(0)) {
                         pyGrid = (int)(p.y + 1);
                         k = flat_index(pyGrid, pxGrid, b.x0, b.y0, n, nn);
                         if (k < 0) ok = false;
-                        else if (C[k] > T(0)) pxGrid = (int)p.x;
+                        else if (C[(k / n) * cp + k % n] > T(0)) pxGrid = (int)p.x;
                     }
                 }
                 if (ok) {
@@ -625,9 +776,9 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             }
             __syncthreads();
             if (threadIdx.x == 0) {
+                // first-encounter order of the neighbour ids, lengths summed per id in contour
+                // order; the list lives in LDS and is written out once
                 int cnt2 = 0;
-                int* ids = A.O.nb_id + (size_t)g * kNbCap;
-                double* lens = A.O.nb_len + (size_t)g * kNbCap;
                 int stop = len - 1, over = 0;
                 for (int i = 0; i < len - 1; ++i) {
                     int cd = s_cand[i];
@@ -635,17 +786,19 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                     double ll = s_len[i] * G.h;
                     int e = 0;
                     for (; e < cnt2; ++e)
-                        if (ids[e] == cd) break;
+                        if (s_nbid[e] == cd) break;
                     if (e == cnt2) {
-                        if (cnt2 < kNbCap) { ids[cnt2] = cd; lens[cnt2] = 0.0; cnt2++; }
+                        if (cnt2 < kNbCap) { s_nbid[cnt2] = cd; s_nblen[cnt2] = 0.0; cnt2++; }
                         else { over = 1; e = -1; }
                     }
-                    if (e >= 0) lens[e] += ll;
+                    if (e >= 0) s_nblen[e] += ll;
                 }
+                s_int[7] = cnt2;
                 A.O.nb_cnt[g] = cnt2;
                 if (over) A.O.err[g] = kErrNbCap;
                 if (stop < len - 1) A.O.flags[g] &= ~kRegular;  // markAsInvalidMotion
                 double e = 0;
+#pragma unroll 8
                 for (int i = 0; i < len - 1; ++i) e += s_len[i] * (i < stop ? 1.0 : -1.0) * G.h;
                 A.O.energy[g] = e;
                 double newVolume = fabs(fabs(s_dbl[0]) * G.h * G.h * 0.5);
@@ -659,6 +812,11 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                 if ((unsigned long)(lp - 1u) % at == 0 || lp == 0) md = 0;
                 A.O.meanda[g] = md + dA;
             }
+            __syncthreads();
+            for (int e = threadIdx.x; e < s_int[7]; e += blockDim.x) {
+                A.O.nb_id[(size_t)g * kNbCap + e] = s_nbid[e];
+                A.O.nb_len[(size_t)g * kNbCap + e] = s_nblen[e];
+            }
         }
         __syncthreads();
         // ---------------- redistancing ----------------
@@ -666,10 +824,10 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const Box bo = s_nbx;
             T* gout = A.out_field + s_onext;
             if constexpr (FULL) {
-                if (bo.n <= nb + 8) redistance<T>(Ol, bo.n | 1, C, b, bo, G, gout);
-                else redistance<T>(gout, bo.n, C, b, bo, G, gout);
+                if (bo.n <= nb + 8) redistance<T>(Ol, bo.n | 1, C, cp, b, bo, G, gout);
+                else redistance<T>(gout, bo.n, C, cp, b, bo, G, gout);
             } else {
-                redistance<T>(gout, bo.n, C, b, bo, G, gout);
+                redistance<T>(gout, bo.n, C, cp, b, bo, G, gout);
             }
         }
         __syncthreads();

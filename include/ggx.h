@@ -129,9 +129,8 @@ typedef struct ggx_ctx ggx_ctx;
 #define GGX_BATCH_FUSED    0  /* all grains per launch (the batched step) */
 #define GGX_BATCH_PERGRAIN 1  /* one launch per grain per kernel (the per-grain baseline) */
 
-#define GGX_CONV_AUTO    0    /* autotuned choice per box size */
-#define GGX_CONV_STENCIL 1    /* exact banded circulant (real space, separable) */
-#define GGX_CONV_FFT     2    /* LDS-staged mixed-radix FFT */
+#define GGX_CONV_AUTO      0  /* the engine chosen per box size */
+#define GGX_CONV_CIRCULANT 1  /* exact dense circulant, K_n phi K_n (real space, separable) */
 
 typedef struct ggx_options {
     int32_t precision;        /* GGX_PREC_* */
@@ -200,16 +199,18 @@ int ggx_get_neighbours(ggx_ctx* ctx, int32_t id, int32_t* ids, double* lengths, 
 /* Comparison list in list order (LSbox::m_comparisonList). */
 int ggx_get_compare_list(ggx_ctx* ctx, int32_t id, int32_t* ids, int32_t cap);
 
-/* Per-phase timers in the reference's stdout categories (grainhdl.cpp:1144-1154), in ms,
- * accumulated over all steps, plus the work and traffic counters of the last step. */
+/* Per-phase timers in the reference's stdout categories (grainhdl.cpp:1144-1154), in ms, and
+ * work / traffic counters, all accumulated over every step of the context. */
 typedef struct ggx_stats {
     double  convolution_ms, comparison_ms, levelset_ms, redistancing_ms, rest_ms, total_ms;
     int64_t steps;
-    int64_t grid_point_updates;   /* sum over grains of n_g^2 in the last step */
-    int64_t algorithmic_bytes;    /* B_step of the last step */
-    int64_t conv_bytes;           /* bytes the convolution kernel must move in the last step */
-    int32_t live_grains;
-    int32_t max_box;
+    int64_t grid_point_updates;   /* sum over steps and live grains of n_g^2 */
+    int64_t algorithmic_bytes;    /* compulsory HBM bytes of all steps: convolution (2 S n^2),
+                                     comparison (S n^2 + S * neighbour overlap), IDLocal (4 n^2),
+                                     redistanced field (S n'^2) */
+    int64_t conv_bytes;           /* the convolution's part of algorithmic_bytes */
+    int32_t live_grains;          /* live grains of the last step */
+    int32_t max_box;              /* largest box side of the last step */
 } ggx_stats;
 int ggx_get_stats(ggx_ctx* ctx, ggx_stats* out);
 

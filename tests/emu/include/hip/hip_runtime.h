@@ -29,10 +29,12 @@ struct dim3 {
     unsigned x, y, z;
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
-struct double2 { double x, y; };
+struct alignas(16) double2 { double x, y; };
 struct int4 { int x, y, z, w; };
 inline double2 make_double2(double a, double b) { return double2{a, b}; }
 inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
+struct alignas(16) float4 { float x, y, z, w; };
+inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
@@ -53,17 +55,51 @@ inline std::function<void()>* g_body = nullptr;
 inline unsigned char* g_dyn = nullptr;
 constexpr size_t kStack = 256 * 1024;
 
+// Barriers: a workgroup barrier waits until every live lane of the block has arrived; a wave
+// barrier (used by the cross-lane operations) until every lane of the caller's 64-lane wave has.
+inline unsigned g_nt = 0, g_done = 0;
+inline unsigned g_bar_count = 0, g_bar_gen = 0;
+inline unsigned g_wave_count[16], g_wave_gen[16];
+inline unsigned long long g_wave_val[16][64];
+inline void yield() { swapcontext(&g_fibers[g_cur].ctx, &g_main); }
 inline void entry() {
     (*g_body)();
     g_fibers[g_cur].done = true;
+    g_done++;
+    if (g_bar_count > 0 && g_bar_count == g_nt - g_done) { g_bar_count = 0; g_bar_gen++; }
     swapcontext(&g_fibers[g_cur].ctx, &g_main);
 }
-inline void sync() { swapcontext(&g_fibers[g_cur].ctx, &g_main); }
+inline void sync() {
+    unsigned gen = g_bar_gen;
+    if (++g_bar_count == g_nt - g_done) { g_bar_count = 0; g_bar_gen++; }
+    while (g_bar_gen == gen) yield();
+}
+inline void wave_sync() {
+    unsigned w = (unsigned)g_cur / 64u, lanes = std::min(64u, g_nt - 64u * w);
+    unsigned gen = g_wave_gen[w];
+    if (++g_wave_count[w] == lanes) { g_wave_count[w] = 0; g_wave_gen[w]++; }
+    while (g_wave_gen[w] == gen) yield();
+}
+inline unsigned long long wave_exchange(unsigned long long v, int src, bool ballot) {
+    unsigned w = (unsigned)g_cur / 64u, lanes = std::min(64u, g_nt - 64u * w);
+    g_wave_val[w][g_cur % 64] = v;
+    wave_sync();
+    unsigned long long r = 0;
+    if (ballot) {
+        for (unsigned l = 0; l < lanes; ++l)
+            if (g_wave_val[w][l]) r |= 1ull << l;
+    } else {
+        r = g_wave_val[w][src & 63];
+    }
+    wave_sync();
+    return r;
+}
 
 inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> body) {
     unsigned nt = block.x * block.y * block.z;
     while (g_stacks.size() < nt) g_stacks.push_back((char*)std::malloc(kStack));
     g_fibers.assign(nt, Fiber{});
+    g_nt = nt;
     g_body = &body;
     g_gdim = grid;
     g_bdim = block;
@@ -72,6 +108,9 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> bo
         std::free(g_dyn);
         g_dyn = (unsigned char*)std::malloc(shmem ? shmem : 1);
         std::memset(g_dyn, 0xCD, shmem ? shmem : 1);
+        g_done = 0;
+        g_bar_count = 0;
+        for (unsigned w = 0; w < 16; ++w) g_wave_count[w] = 0;
         for (unsigned t = 0; t < nt; ++t) {
             getcontext(&g_fibers[t].ctx);
             g_fibers[t].ctx.uc_stack.ss_sp = g_stacks[t];
@@ -96,6 +135,11 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> bo
 }
 }  // namespace emu
 
+// every lane of the emulator's workgroups holds the same value wherever the kernels use this
+inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+inline int __builtin_amdgcn_readlane(int v, int src) { return (int)emu::wave_exchange((unsigned)v, src, false); }
+inline unsigned long long __ballot(int p) { return emu::wave_exchange(p ? 1ull : 0ull, 0, true); }
+
 #define threadIdx (emu::g_tid)
 #define blockIdx (emu::g_bid)
 #define blockDim (emu::g_bdim)
@@ -107,6 +151,7 @@ inline void __threadfence_block() {}
 inline int __ffs(unsigned int v) { return __builtin_ffs((int)v); }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) { auto o = *p; *p += v; return o; }
 inline int atomicAdd(int* p, int v) { int o = *p; *p += v; return o; }
+inline unsigned atomicAdd(unsigned* p, unsigned v) { unsigned o = *p; *p += v; return o; }
 inline int atomicMin(int* p, int v) { int o = *p; if (v < o) *p = v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 inline int atomicOr(int* p, int v) { int o = *p; *p |= v; return o; }
