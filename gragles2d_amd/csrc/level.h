@@ -193,14 +193,25 @@ __device__ __forceinline__ void wave_trace(const WaveMask& M, int n, int x0, int
         if (lane == 0) s_int[0] = 0x7fffffff;
         return;
     }
-    const unsigned long long a = readlane64(r < 64 ? M.m00 : M.m10, r & 63);
-    const unsigned long long b = readlane64(r < 64 ? M.m01 : M.m11, r & 63);
+    // the four words stay in named registers; every choice among them is a uniform branch
+    // around a readlane, so no select between their addresses can send them to scratch
+    const unsigned long long m00 = M.m00, m01 = M.m01, m10 = M.m10, m11 = M.m11;
+    unsigned long long a, b;
+    if (r < 64) { a = readlane64(m00, r); b = readlane64(m01, r); }
+    else { a = readlane64(m10, r - 64); b = readlane64(m11, r - 64); }
     const int c = a ? __builtin_ctzll(a) : 64 + __builtin_ctzll(b);
-    auto inside = [&](int row, int col) -> bool {
+    auto inside = [=](int row, int col) -> bool {
         const int rr = row - y0, q = col - x0;
         if ((unsigned)rr >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
-        const unsigned long long v = rr < 64 ? (q < 64 ? M.m00 : M.m01) : (q < 64 ? M.m10 : M.m11);
-        return (readlane64(v, rr & 63) >> (q & 63)) & 1ull;
+        unsigned long long w;
+        if (rr < 64) {
+            if (q < 64) w = readlane64(m00, rr);
+            else w = readlane64(m01, rr);
+        } else {
+            if (q < 64) w = readlane64(m10, rr - 64);
+            else w = readlane64(m11, rr - 64);
+        }
+        return (w >> (q & 63)) & 1ull;
     };
     auto emit = [&](int k, uint32_t mv) {
         if (lane == 0) moves[k] = mv;
