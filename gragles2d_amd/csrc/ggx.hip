@@ -50,6 +50,12 @@ struct ggx_ctx {
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
+    // side streams (default priority) that run the bucket launches of a phase concurrently, so
+    // that the few-grain buckets of large boxes overlap the many-grain ones; fork/join events
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev[2] = {nullptr, nullptr};
+    hipEvent_t join_ev[2][kSide] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     int lds_max = 65536;
     int cu_count = 0;
     bool loaded = false;
@@ -264,13 +270,34 @@ std::vector<hipEvent_t> take_events(ggx_ctx* ctx, int count) {
     return ev;
 }
 
+// Work of phase `ph` may run on the side streams once the main stream reaches this point.
+int fork(ggx_ctx* ctx, int ph) {
+    HIPCHK(hipEventRecord(ctx->fork_ev[ph], ctx->stream));
+    for (int k = 0; k < ggx_ctx::kSide; ++k) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev[ph], 0));
+    return GGX_OK;
+}
+
+// The main stream continues once the side streams have finished phase `ph`.
+int join(ggx_ctx* ctx, int ph) {
+    for (int k = 0; k < ggx_ctx::kSide; ++k) {
+        HIPCHK(hipEventRecord(ctx->join_ev[ph][k], ctx->side[k]));
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[ph][k], 0));
+    }
+    return GGX_OK;
+}
+
+// Stream of the q-th launch of a phase: round robin over the main and side streams.
+hipStream_t lane_stream(const ggx_ctx* ctx, int q) {
+    const int k = q % (ggx_ctx::kSide + 1);
+    return k == 0 ? ctx->stream : ctx->side[k - 1];
+}
+
 // Convolution of the grains of one work list whose boxes have side <= nb: the register-blocked
 // kernel when both of its tiles fit in LDS, else the strip kernel.
 template <class T>
-int launch_conv(ggx_ctx* ctx, const int* list, const int* count, int* cursor, int nb, int grid, const T* src,
-                T* dst) {
+int launch_conv(ggx_ctx* ctx, hipStream_t st, const int* list, const int* count, int* cursor, int nb, int grid,
+                const T* src, T* dst) {
     const int ti = sizeof(T) == 8 ? 0 : 1;
-    hipStream_t st = ctx->stream;
     const long long dense = dense_lds<T>(nb);
     if (dense + ctx->static_dense[ti] + 256 <= ctx->lds_max) {
         if (grid <= 0) grid = ctx->cu_count * per_cu(ctx, dense + ctx->static_dense[ti]);
@@ -309,7 +336,7 @@ int launch_step(ggx_ctx* ctx) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(32), 0, st, S);
     hipLaunchKernelGGL(k_prep, dim3((NS + 255) / 256), dim3(256), 0, st, NS, ctx->flags, ctx->box, bucket_table(),
                        ctx->blists, ctx->alist, ctx->errs, S);
-    hipLaunchKernelGGL(k_lists, dim3(cus * 4), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
+    hipLaunchKernelGGL(k_lists, dim3(cus * 8), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
                        ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
     HIPCHK(hipEventRecord(ev[1], st));
     // grid per bucket: persistent (fixed) or one workgroup per grain (counts read back)
@@ -319,14 +346,17 @@ int launch_step(ggx_ctx* ctx) {
         HIPCHK(hipStreamSynchronize(st));
         for (int b = 0; b < kNumBuckets; ++b) counts[b] = ctx->h_sched->bucket_count[b];
     }
-    // convolution, one launch per bucket
-    for (int b = 0; b < kNumBuckets; ++b) {
+    // convolution, one launch per bucket, largest boxes first, spread over the main and side streams
+    int rc = fork(ctx, 0);
+    if (rc) return rc;
+    for (int b = kNumBuckets - 1, q = 0; b >= 0; --b) {
         int grid = ctx->opt.persistent ? 0 : counts[b];
         if (!ctx->opt.persistent && grid == 0) continue;
-        int rc = launch_conv<T>(ctx, ctx->blists + (size_t)b * NS, &S->bucket_count[b], &S->bucket_cursor[b],
-                                kBucketEdges[b], grid, (const T*)P, Tt);
+        rc = launch_conv<T>(ctx, lane_stream(ctx, q++), ctx->blists + (size_t)b * NS, &S->bucket_count[b],
+                            &S->bucket_cursor[b], kBucketEdges[b], grid, (const T*)P, Tt);
         if (rc) return rc;
     }
+    if ((rc = join(ctx, 0))) return rc;
     HIPCHK(hipEventRecord(ev[2], st));
     // fused comparison / level set / redistancing, one persistent launch per bucket
     ContourOut O;
@@ -335,11 +365,13 @@ int launch_step(ggx_ctx* ctx) {
     O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
     O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
     O.tops = &S->tops[0]; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
-    for (int b = 0; b < kNumBuckets; ++b) {
+    if ((rc = fork(ctx, 1))) return rc;
+    for (int b = kNumBuckets - 1, q = 0; b >= 0; --b) {
         int nb = kBucketEdges[b];
         LevelLayout full = level_layout<T>(nb, true), small = level_layout<T>(nb, false);
         long long lds = full.total <= level_max ? full.total : small.total;
-        if (lds > level_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel");
+        if (lds > level_max)
+            return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel: bucket " + std::to_string(nb) + " needs " + std::to_string(lds) + " of " + std::to_string(level_max) + " LDS bytes");
         LevelArgs<T> A;
         A.list = ctx->blists + (size_t)b * NS;
         A.count = &S->bucket_count[b];
@@ -353,9 +385,11 @@ int launch_step(ggx_ctx* ctx) {
         int grid = cus * per_cu(ctx, lds + ctx->static_level[ti]);
         if (!ctx->opt.persistent) grid = counts[b];
         if (grid == 0) continue;
-        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(grid), dim3(256), (size_t)lds, st, A);
-        else hipLaunchKernelGGL((k_level<T, false>), dim3(grid), dim3(256), (size_t)lds, st, A);
+        hipStream_t ls = lane_stream(ctx, q++);
+        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(grid), dim3(256), (size_t)lds, ls, A);
+        else hipLaunchKernelGGL((k_level<T, false>), dim3(grid), dim3(256), (size_t)lds, ls, A);
     }
+    if ((rc = join(ctx, 1))) return rc;
     HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK(hipGetLastError());
     ctx->ev_pending.push_back(ev);
@@ -419,7 +453,7 @@ int launch_step_pergrain(ggx_ctx* ctx) {
                            ctx->box, ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
     HIPCHK(hipEventRecord(ev[1], st));
     for (size_t i = 0; i < live.size(); ++i) {
-        int rc = launch_conv<T>(ctx, ctx->pg_list + i, ctx->pg_one + i, ctx->pg_cursor + L + i, bx[live[i]].n, 1,
+        int rc = launch_conv<T>(ctx, st, ctx->pg_list + i, ctx->pg_one + i, ctx->pg_cursor + L + i, bx[live[i]].n, 1,
                                 (const T*)P, Tt);
         if (rc) return rc;
     }
@@ -434,7 +468,8 @@ int launch_step_pergrain(ggx_ctx* ctx) {
         int n = bx[live[i]].n;
         LevelLayout full = level_layout<T>(n, true), small = level_layout<T>(n, false);
         long long lds = full.total <= level_max ? full.total : small.total;
-        if (lds > level_max) return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel");
+        if (lds > level_max)
+            return fail(ctx, GGX_E_BOUNDS, "box too large for the level-set kernel: bucket " + std::to_string(n) + " needs " + std::to_string(lds) + " of " + std::to_string(level_max) + " LDS bytes");
         LevelArgs<T> A;
         A.list = ctx->pg_list + i;
         A.count = ctx->pg_one + i;
@@ -488,6 +523,13 @@ extern "C" int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx**
     hipError_t e = hipSetDevice(ctx->dev);
     if (e != hipSuccess) { delete ctx; return GGX_E_HIP; }
     if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return GGX_E_HIP; }
+    for (int k = 0; k < ggx_ctx::kSide; ++k)
+        if (hipStreamCreate(&ctx->side[k]) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
+    for (int ph = 0; ph < 2; ++ph) {
+        if (hipEventCreateWithFlags(&ctx->fork_ev[ph], hipEventDisableTiming) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
+        for (int k = 0; k < ggx_ctx::kSide; ++k)
+            if (hipEventCreateWithFlags(&ctx->join_ev[ph][k], hipEventDisableTiming) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
+    }
     if (raise_lds(ctx) != GGX_OK) { delete ctx; return GGX_E_HIP; }
     std::memset(&ctx->st, 0, sizeof(ctx->st));
     ctx->loop = s->start_time;
@@ -509,6 +551,13 @@ extern "C" void ggx_destroy(ggx_ctx* ctx) {
     if (ctx->h_sched) (void)hipHostFree(ctx->h_sched);
     for (auto& v : ctx->ev_pending) for (auto e : v) (void)hipEventDestroy(e);
     for (auto& v : ctx->ev_free) for (auto e : v) (void)hipEventDestroy(e);
+    for (int ph = 0; ph < 2; ++ph) {
+        if (ctx->fork_ev[ph]) (void)hipEventDestroy(ctx->fork_ev[ph]);
+        for (int k = 0; k < ggx_ctx::kSide; ++k)
+            if (ctx->join_ev[ph][k]) (void)hipEventDestroy(ctx->join_ev[ph][k]);
+    }
+    for (int k = 0; k < ggx_ctx::kSide; ++k)
+        if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -22,6 +22,7 @@ struct Box {
 
 // Limits of the fixed-capacity per-grain lists on the device.
 constexpr int kCmpCap = 128;     // comparison list (m_comparisonList / m_secondOrderNeighbours)
+static_assert(kCmpCap <= 256, "list positions are packed in 8 bits");
 constexpr int kNbCap = 64;       // direct neighbours (m_directNeighbourhood)
 constexpr int kAlign = 64;       // element alignment of every box in the arenas
 

@@ -44,22 +44,6 @@ struct LevelArgs {
     int lds_bytes;
 };
 
-// Block-wide exclusive scan of one int per lane (blockDim.x <= 1024); returns the total.
-__device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp) {
-    int t = threadIdx.x;
-    s_tmp[t] = v;
-    __syncthreads();
-    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-        int add = (t >= d) ? s_tmp[t - d] : 0;
-        __syncthreads();
-        s_tmp[t] += add;
-        __syncthreads();
-    }
-    int incl = s_tmp[t];
-    __syncthreads();
-    return incl - v;
-}
-
 // Interpolated point of a recorded move (marchingSquares.cpp:204-265).
 template <class T>
 __device__ __forceinline__ double2 move_point(const T* C, int n, int cp, int x0, int y0, uint32_t mv) {
@@ -178,6 +162,43 @@ __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, i
     return ((unsigned long long)hi << 32) | lo;
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+    return __longlong_as_double((long long)readlane64((unsigned long long)__double_as_longlong(v), src));
+}
+
+// Sum of arr[0..len) in index order (the reference's left-to-right accumulation), by one whole
+// wave: 64 values per LDS read, then one readlane and one add per term.  len must be the same
+// in every lane; every lane returns the sum.
+__device__ __forceinline__ double wave_ordered_sum(const double* arr, int len) {
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int c0 = 0; c0 < len; c0 += 64) {
+        const double v = (c0 + lane < len) ? arr[c0 + lane] : 0.0;
+        const int m = min(64, len - c0);
+        for (int k = 0; k < m; ++k) s += readlane_f64(v, k);
+    }
+    return s;
+}
+
+// Exclusive prefix count of a per-lane flag over the workgroup in lane order, and the total:
+// one ballot per wave and one exchange of wave totals through LDS (s_wt, one int per wave).
+__device__ __forceinline__ int block_flag_scan(bool f, int* s_wt, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long m = __ballot(f);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wt[wave] = __popcll(m);
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int c = s_wt[w];
+        if (w < wave) base += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + pre;
+}
+
 // Wave 0 of the workgroup: start pixel (first inside pixel in row-major order) and walk.
 // Writes s_int[0] (start index or INT_MAX when nothing is inside), s_int[1] (moves) and, on
 // failure, s_int[3].
@@ -225,26 +246,25 @@ __device__ __forceinline__ void wave_trace(const WaveMask& M, int n, int x0, int
     }
 }
 
-// LDS layout of the fused kernel for boxes up to side nb.  Full layout: C tile, O tile (which
-// also hosts the per-point work arrays until redistancing), moves, scan scratch.  Small
-// layout (boxes too large for LDS): C and O stay in global memory, only the work arrays,
-// moves and scan scratch are in LDS.
+// LDS layout of the fused kernel for boxes up to side nb.  Full layout: the C tile, then the O
+// tile, which until redistancing hosts the per-point work arrays (36 bytes per contour point:
+// the compacted contour; the interpolated moves, later the segment lengths and shoelace
+// terms; the recorded moves, later the neighbour probes).  Small layout (boxes too large for
+// LDS): C and O stay in global memory, only the work arrays are in LDS.
 struct LevelLayout {
-    long long ctile, otile, moves, scan, total;
+    long long ctile, otile, total;
 };
 
 template <class T>
 __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     LevelLayout L;
-    long long ccap = contour_cap(nb);
-    long long no = nb + 8;
-    long long work = (ccap * 20 + 15) & ~15LL;
+    const long long ccap = contour_cap(nb);
+    const long long no = nb + 8;
+    const long long work = (ccap * 36 + 15) & ~15LL;
     L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
     L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
     if (work > L.otile) L.otile = work;
-    L.moves = (ccap * 4 + 15) & ~15LL;
-    L.scan = 1024 * 4;
-    L.total = L.ctile + L.otile + L.moves + L.scan;
+    L.total = L.ctile + L.otile;
     return L;
 }
 
@@ -392,22 +412,22 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
     T* Cl = (T*)smem;
     unsigned char* obase = smem + LY.ctile;
     T* Ol = (T*)obase;
-    double* s_len = (double*)obase;                       // aliases O until redistancing
+    double2* s_pt = (double2*)obase;        // work arrays (alias O until redistancing)
+    double2* s_raw = s_pt + ccap;
+    double* s_len = (double*)s_raw;         // s_len, s_term reuse s_raw after the compaction
     double* s_term = s_len + ccap;
-    int* s_cand = (int*)(s_term + ccap);
-    uint32_t* moves = (uint32_t*)(obase + LY.otile);
-    int* s_scan = (int*)((unsigned char*)moves + LY.moves);
+    int* s_cand = (int*)(s_raw + ccap);
+    uint32_t* moves = (uint32_t*)s_cand;    // moves are dead before the probes are written
 
     __shared__ int s_bb[4];
-    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len, 7 neighbours
+    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len
     __shared__ double s_dbl[8];   // 0 volume sum, 1 minx, 2 maxx, 3 miny, 4 maxy
     __shared__ int s_id[kCmpCap], s_x0[kCmpCap], s_y0[kCmpCap], s_n[kCmpCap];
     __shared__ long long s_off[kCmpCap];
     __shared__ long long s_coff, s_onext;
     __shared__ Box s_nbx;
     __shared__ unsigned s_hits;
-    __shared__ int s_nbid[kNbCap];
-    __shared__ double s_nblen[kNbCap];
+    __shared__ int s_wt[16];
 
     __shared__ int s_slot;
     while (true) {
@@ -449,37 +469,64 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const double band = 0.7 * G.delta;
             int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
             unsigned hits = 0;
-            // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
-            // chunk, so the gathers of one neighbour are independent loads issued together;
-            // per pixel the neighbours are still met in list order (strict >, first wins).
-            constexpr int kPx = 8;
+            // Each lane keeps kPx pixels of a chunk in registers.  Pass 1 (ALU only) records, per
+            // pixel, the list positions of the first kHit neighbours whose boxes cover it; pass 2
+            // issues all their gathers at once and takes the maximum in list order (strict >,
+            // first wins); pixels covered by more neighbours finish the list one by one.
+            constexpr int kPx = 4, kHit = 4;
             for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
-                int yy[kPx], xx[kPx], id[kPx];
+                int yy[kPx], xx[kPx], id[kPx], nh[kPx];
+                uint32_t hl[kPx];
                 T mx[kPx];
 #pragma unroll
                 for (int t = 0; t < kPx; ++t) {
                     const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
-                    yy[t] = p < nn ? (int)(p / n) : n;  // row n lies outside every overlap
+                    yy[t] = p < nn ? (int)(p / n) : -(1 << 20);  // a row no box covers
                     xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
                     mx[t] = T(-1.0);
                     id[t] = -1;
+                    nh[t] = 0;
+                    hl[t] = 0u;
                 }
-#pragma unroll 2
+                const int r0 = (int)(c0 / n), r1 = (int)min((long long)n - 1, (c0 + (long long)kPx * blockDim.x - 1) / n);
                 for (int k = 0; k < cnt; ++k) {
                     const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
-                    const T* nsrc = A.tilde + s_off[k];
-                    T v[kPx];
-                    bool in[kPx];
+                    if (dy > r1 || dy + nk <= r0) continue;  // the neighbour misses every row of the chunk
 #pragma unroll
                     for (int t = 0; t < kPx; ++t) {
-                        const int ly = yy[t] - dy, lx = xx[t] - dx;
-                        in[t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
-                        v[t] = in[t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                        const bool in = (unsigned)(yy[t] - dy) < (unsigned)nk && (unsigned)(xx[t] - dx) < (unsigned)nk;
+                        if (in) {
+                            if (nh[t] < kHit) hl[t] |= (uint32_t)k << (8 * nh[t]);
+                            nh[t]++;
+                        }
+                    }
+                }
+                T v[kHit][kPx];
+#pragma unroll
+                for (int hh = 0; hh < kHit; ++hh)
+#pragma unroll
+                    for (int t = 0; t < kPx; ++t) {
+                        v[hh][t] = T(0);
+                        if (hh < nh[t]) {
+                            const int k = (int)((hl[t] >> (8 * hh)) & 255u);
+                            const int nk = s_n[k];
+                            v[hh][t] = A.tilde[s_off[k] + (long long)(yy[t] + b.y0 - s_y0[k]) * nk + (xx[t] + b.x0 - s_x0[k])];
+                        }
                     }
 #pragma unroll
-                    for (int t = 0; t < kPx; ++t) {
-                        hits += in[t] ? 1u : 0u;
-                        if (in[t] && v[t] > mx[t]) { mx[t] = v[t]; id[t] = s_id[k]; }
+                for (int t = 0; t < kPx; ++t) {
+                    hits += (unsigned)nh[t];
+#pragma unroll
+                    for (int hh = 0; hh < kHit; ++hh)
+                        if (hh < nh[t] && v[hh][t] > mx[t]) { mx[t] = v[hh][t]; id[t] = s_id[(hl[t] >> (8 * hh)) & 255u]; }
+                    if (nh[t] > kHit) {  // rare: the remaining covering neighbours in list order
+                        for (int k = (int)(hl[t] >> 24) + 1; k < cnt; ++k) {
+                            const int ly = yy[t] + b.y0 - s_y0[k], lx = xx[t] + b.x0 - s_x0[k], nk = s_n[k];
+                            if ((unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk) {
+                                const T w = A.tilde[s_off[k] + (long long)ly * nk + lx];
+                                if (w > mx[t]) { mx[t] = w; id[t] = s_id[k]; }
+                            }
+                        }
                     }
                 }
 #pragma unroll
@@ -580,90 +627,87 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                 __syncthreads();
                 break;
             }
-            // all lanes: interpolate, drop consecutive duplicates (insertPoint,
-            // marchingSquares.cpp:197-203), compact in order
-            const int nm = s_int[1];
-            const int per = (nm + blockDim.x - 1) / blockDim.x;
-            const int k0 = threadIdx.x * per, k1 = min(nm, k0 + per);
-            int keep_cnt = 0;
-            for (int k = k0; k < k1; ++k) {
-                double2 p = move_point(C, n, cp, b.x0, b.y0, moves[k]);
-                bool keep = true;
-                if (k > 0) {
-                    double2 q = move_point(C, n, cp, b.x0, b.y0, moves[k - 1]);
-                    keep = (p.x != q.x || p.y != q.y);
-                }
-                keep_cnt += keep ? 1 : 0;
-            }
-            int base = block_exclusive_scan(keep_cnt, s_scan);
-            int w = base;
-            for (int k = k0; k < k1; ++k) {
-                double2 p = move_point(C, n, cp, b.x0, b.y0, moves[k]);
-                bool keep = true;
-                if (k > 0) {
-                    double2 q = move_point(C, n, cp, b.x0, b.y0, moves[k - 1]);
-                    keep = (p.x != q.x || p.y != q.y);
-                }
-                if (keep) {
-                    if (w < ccap) pts[w] = p;
-                    w++;
-                }
-            }
-            if (k1 == nm && k0 < k1) s_int[6] = w;  // lane holding the last move knows the count
-            if (nm == 0 && threadIdx.x == 0) s_int[6] = 0;
-            __threadfence_block();
+            // all lanes: interpolate the recorded moves, drop consecutive duplicates (insertPoint,
+            // marchingSquares.cpp:197-203) and compact in order into s_pt and the contour arena
+            const int nm = __builtin_amdgcn_readfirstlane(s_int[1]);
+            for (int k = threadIdx.x; k < nm; k += blockDim.x) s_raw[k] = move_point(C, n, cp, b.x0, b.y0, moves[k]);
             __syncthreads();
-            if (threadIdx.x == 0) {
-                int len = s_int[6];
-                if (len > 0) {
-                    double2 f = pts[0], l = pts[len - 1];
-                    if (f.x != l.x || f.y != l.y) {
-                        if (len < ccap) pts[len] = f;
-                        len++;
+            int kept = 0;
+            for (int r0 = 0; r0 < nm; r0 += blockDim.x) {
+                const int k = r0 + threadIdx.x;
+                bool keep = false;
+                double2 pk = make_double2(0.0, 0.0);
+                if (k < nm) {
+                    pk = s_raw[k];
+                    keep = true;
+                    if (k > 0) {
+                        const double2 q = s_raw[k - 1];
+                        keep = (pk.x != q.x || pk.y != q.y);
                     }
                 }
+                int tot;
+                const int pos = kept + block_flag_scan(keep, s_wt, &tot);
+                if (keep && pos < ccap) { s_pt[pos] = pk; pts[pos] = pk; }
+                kept += tot;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {  // close the contour (marchingSquares.cpp:180-186)
+                int len = kept;
                 if (len > ccap) s_int[3] = kErrContourCap;
+                else if (len > 0) {
+                    const double2 f = s_pt[0], l = s_pt[len - 1];
+                    if (f.x != l.x || f.y != l.y) {
+                        if (len < ccap) { s_pt[len] = f; pts[len] = f; }
+                        len++;
+                    }
+                    if (len > ccap) s_int[3] = kErrContourCap;
+                }
                 s_int[6] = len;
             }
-            __threadfence_block();
             __syncthreads();
-            const int len = s_int[6];
+            const int len = __builtin_amdgcn_readfirstlane(s_int[6]);
             if (s_int[3]) {
                 if (threadIdx.x == 0) s_int[5] = 0;
                 __syncthreads();
                 break;
             }
-            // shoelace terms and segment lengths, summed in order by lane 0
+            // shoelace terms and segment lengths
             for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
-                double2 a = pts[i], c2 = pts[i + 1];
+                const double2 a = s_pt[i], c2 = s_pt[i + 1];
                 s_term[i] = (a.y + c2.y) * (a.x - c2.x);
-                double dx = a.x - c2.x, dy = a.y - c2.y;
+                const double dx = a.x - c2.x, dy = a.y - c2.y;
                 s_len[i] = sqrt(dx * dx + dy * dy);
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                double vs = 0;
-#pragma unroll 8
-                for (int i = 0; i < len - 1; ++i) vs += s_term[i];
-                s_dbl[0] = vs;
-                double newVolume = fabs(vs) * G.h * G.h * 0.5;
-                double mullins = (newVolume - old_vol) / G.dt * (3 / 3.14159265358979323846);
-                double contourRatio = ((double)len) / old_len;
-                double volumeRatio = newVolume / old_vol;
-                int regular;
+            if (threadIdx.x < 64) {  // computeVolume + isMotionRegular (box.cpp:1155-1190), wave 0
+                const double vs = wave_ordered_sum(s_term, len - 1);
+                const double newVolume = fabs(vs) * G.h * G.h * 0.5;
+                const double mullins = (newVolume - old_vol) / G.dt * (3 / 3.14159265358979323846);
+                const double contourRatio = ((double)len) / old_len;
+                const double volumeRatio = newVolume / old_vol;
+                int regular, flag_off = 0;
                 if (mullins > -6.0) regular = 1;
                 else if (volumeRatio < 0.10) regular = 0;
                 else if (contourRatio < 0.35) regular = 0;
                 else if (contourRatio > 0.7) regular = 1;
-                else { s_int[4] = 0; regular = 1; }
-                s_int[2] = regular ? 2 : 1;
+                else { flag_off = 1; regular = 1; }
+                if (threadIdx.x == 0) {
+                    s_dbl[0] = vs;
+                    if (flag_off) s_int[4] = 0;
+                    s_int[2] = regular ? 2 : 1;
+                }
                 if (!regular) {  // bounding box of the contour for clearContourGrainArea
-                    double mnx = pts[0].x, mxx = mnx, mny = pts[0].y, mxy = mny;
-                    for (int i = 0; i < len; ++i) {
-                        mnx = fmin(mnx, pts[i].x); mxx = fmax(mxx, pts[i].x);
-                        mny = fmin(mny, pts[i].y); mxy = fmax(mxy, pts[i].y);
+                    double mnx = s_pt[0].x, mxx = mnx, mny = s_pt[0].y, mxy = mny;
+                    for (int i = threadIdx.x; i < len; i += 64) {
+                        const double2 q = s_pt[i];
+                        mnx = fmin(mnx, q.x); mxx = fmax(mxx, q.x);
+                        mny = fmin(mny, q.y); mxy = fmax(mxy, q.y);
                     }
-                    s_dbl[1] = mnx; s_dbl[2] = mxx; s_dbl[3] = mny; s_dbl[4] = mxy;
+                    for (int d = 32; d >= 1; d >>= 1) {
+                        mnx = fmin(mnx, __shfl_xor(mnx, d)); mxx = fmax(mxx, __shfl_xor(mxx, d));
+                        mny = fmin(mny, __shfl_xor(mny, d)); mxy = fmax(mxy, __shfl_xor(mxy, d));
+                    }
+                    if (threadIdx.x == 0) { s_dbl[1] = mnx; s_dbl[2] = mxx; s_dbl[3] = mny; s_dbl[4] = mxy; }
                 }
             }
             __syncthreads();
@@ -744,16 +788,13 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             atomicAdd(A.out_total, (unsigned long long)nbx.n * (unsigned long long)nbx.n);
             s_nbx = nbx;
             s_onext = (long long)fo;
-            double per = 0;  // computePerimeter (grainBoundary.cpp:224-231)
-#pragma unroll 8
-            for (int i = 0; i < len - 1; ++i) per += s_len[i] * G.h;
-            A.O.perimeter[g] = per;
         }
         __syncthreads();
         // ---------------- direct neighbours (only after a regular motion) ----------------
-        if (s_int[4]) {
+        const bool nbs = s_int[4] != 0;
+        if (nbs) {  // one probe pixel per contour point (buildDirectNeighbours, grainBoundary.cpp:85-120)
             for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
-                double2 p = pts[i];
+                double2 p = s_pt[i];
                 int pxGrid = (int)p.x, pyGrid = (int)p.y;
                 int cand = -1;
                 bool ok = true;
@@ -785,48 +826,67 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                 }
                 s_cand[i] = ok ? cand : -1;
             }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                // first-encounter order of the neighbour ids, lengths summed per id in contour
-                // order; the list lives in LDS and is written out once
-                int cnt2 = 0;
-                int stop = len - 1, over = 0;
-                for (int i = 0; i < len - 1; ++i) {
-                    int cd = s_cand[i];
-                    if (cd < 0) { stop = i; break; }
-                    double ll = s_len[i] * G.h;
-                    int e = 0;
-                    for (; e < cnt2; ++e)
-                        if (s_nbid[e] == cd) break;
-                    if (e == cnt2) {
-                        if (cnt2 < kNbCap) { s_nbid[cnt2] = cd; s_nblen[cnt2] = 0.0; cnt2++; }
-                        else { over = 1; e = -1; }
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            // wave 0 walks the contour in order: perimeter (computePerimeter,
+            // grainBoundary.cpp:224-231); after a regular motion also the neighbour list in
+            // first-encounter order with its per-neighbour lengths (lane e holds neighbour e),
+            // the stop at the first failed probe, and the energy (computeEnergy, 215-222)
+            const int lane = threadIdx.x & 63;
+            const int L1 = len - 1;
+            double per = 0.0, e = 0.0, my_len = 0.0;
+            int my_id = -1, cnt2 = 0, over = 0, stop = L1;
+            bool stopped = false;
+            for (int c0 = 0; c0 < L1; c0 += 64) {
+                const int i = c0 + lane;
+                const double lv = i < L1 ? s_len[i] : 0.0;
+                const int cv = (nbs && i < L1) ? s_cand[i] : 0;
+                const int m = min(64, L1 - c0);
+                for (int k = 0; k < m; ++k) {
+                    const double l = readlane_f64(lv, k);
+                    per += l * G.h;
+                    if (nbs) {
+                        const int cd = __builtin_amdgcn_readlane(cv, k);
+                        if (!stopped && cd < 0) { stopped = true; stop = c0 + k; }
+                        if (!stopped) {
+                            const double ll = l * G.h;
+                            if (__ballot(my_id == cd) == 0ull) {
+                                if (cnt2 < kNbCap) {
+                                    if (lane == cnt2) { my_id = cd; my_len = 0.0; }
+                                    cnt2++;
+                                } else {
+                                    over = 1;
+                                }
+                            }
+                            if (my_id == cd) my_len += ll;
+                        }
+                        e += l * (stopped ? -1.0 : 1.0) * G.h;
                     }
-                    if (e >= 0) s_nblen[e] += ll;
                 }
-                s_int[7] = cnt2;
-                A.O.nb_cnt[g] = cnt2;
-                if (over) A.O.err[g] = kErrNbCap;
-                if (stop < len - 1) A.O.flags[g] &= ~kRegular;  // markAsInvalidMotion
-                double e = 0;
-#pragma unroll 8
-                for (int i = 0; i < len - 1; ++i) e += s_len[i] * (i < stop ? 1.0 : -1.0) * G.h;
-                A.O.energy[g] = e;
-                double newVolume = fabs(fabs(s_dbl[0]) * G.h * G.h * 0.5);
-                double dA = newVolume - old_vol;
-                dA = dA / G.dt;
-                dA = dA * (3 / 3.14159265358979323846);
-                A.O.volume[g] = newVolume;
-                unsigned int lp = (unsigned int)A.loop;
-                unsigned long at = A.analysis_timestep > 0 ? (unsigned long)A.analysis_timestep : 1ul;
-                double md = A.O.meanda[g];
-                if ((unsigned long)(lp - 1u) % at == 0 || lp == 0) md = 0;
-                A.O.meanda[g] = md + dA;
             }
-            __syncthreads();
-            for (int e = threadIdx.x; e < s_int[7]; e += blockDim.x) {
-                A.O.nb_id[(size_t)g * kNbCap + e] = s_nbid[e];
-                A.O.nb_len[(size_t)g * kNbCap + e] = s_nblen[e];
+            if (lane == 0) A.O.perimeter[g] = per;
+            if (nbs) {
+                if (lane < cnt2) {
+                    A.O.nb_id[(size_t)g * kNbCap + lane] = my_id;
+                    A.O.nb_len[(size_t)g * kNbCap + lane] = my_len;
+                }
+                if (lane == 0) {
+                    A.O.nb_cnt[g] = cnt2;
+                    if (over) A.O.err[g] = kErrNbCap;
+                    if (stop < L1) A.O.flags[g] &= ~kRegular;  // markAsInvalidMotion
+                    A.O.energy[g] = e;
+                    double newVolume = fabs(fabs(s_dbl[0]) * G.h * G.h * 0.5);
+                    double dA = newVolume - old_vol;
+                    dA = dA / G.dt;
+                    dA = dA * (3 / 3.14159265358979323846);
+                    A.O.volume[g] = newVolume;
+                    unsigned int lp = (unsigned int)A.loop;
+                    unsigned long at = A.analysis_timestep > 0 ? (unsigned long)A.analysis_timestep : 1ul;
+                    double md = A.O.meanda[g];
+                    if ((unsigned long)(lp - 1u) % at == 0 || lp == 0) md = 0;
+                    A.O.meanda[g] = md + dA;
+                }
             }
         }
         __syncthreads();

@@ -124,6 +124,13 @@ int main(int argc, char** argv) {
             }
             k1 = ggx_get_compare_list(ctx, id, na.data(), 4096);
             k2 = ggxo_grain_compare_list(oc, id, nb.data());
+            if (k1 != k2 && getenv("GGX_SHOW_CMP")) {
+                printf("  g%d cmp gpu:", id);
+                for (int k = 0; k < k1; ++k) printf(" %d", na[k]);
+                printf("\n  g%d cmp orc:", id);
+                for (int k = 0; k < k2; ++k) printf(" %d", nb[k]);
+                printf("\n");
+            }
             if (k1 != k2) cmpmis++;
             else for (int k = 0; k < k1; ++k) if (na[k] != nb[k]) { cmpmis++; break; }
         }

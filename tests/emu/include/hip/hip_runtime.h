@@ -135,10 +135,17 @@ inline void launch(dim3 grid, dim3 block, size_t shmem, std::function<void()> bo
 }
 }  // namespace emu
 
-// every lane of the emulator's workgroups holds the same value wherever the kernels use this
-inline int __builtin_amdgcn_readfirstlane(int v) { return v; }
+inline void __builtin_amdgcn_wave_barrier() { emu::wave_sync(); }
+inline int __builtin_amdgcn_readfirstlane(int v) { return (int)emu::wave_exchange((unsigned)v, 0, false); }
 inline int __builtin_amdgcn_readlane(int v, int src) { return (int)emu::wave_exchange((unsigned)v, src, false); }
 inline unsigned long long __ballot(int p) { return emu::wave_exchange(p ? 1ull : 0ull, 0, true); }
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
+inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
+inline double __shfl_xor(double v, int mask) {
+    return __longlong_as_double((long long)emu::wave_exchange((unsigned long long)__double_as_longlong(v),
+                                                             (emu::g_cur % 64) ^ mask, false));
+}
 
 #define threadIdx (emu::g_tid)
 #define blockIdx (emu::g_bid)
@@ -188,6 +195,9 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new double(0); return 0; }
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
+constexpr unsigned hipEventDisableTiming = 2;
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = new double(0); return 0; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return 0; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
     *e = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
     return 0;
