@@ -50,12 +50,6 @@ struct ggx_ctx {
     std::string err;
     int dev = 0;
     hipStream_t stream = nullptr;
-    // side streams (default priority) that run the bucket launches of a phase concurrently, so
-    // that the few-grain buckets of large boxes overlap the many-grain ones; fork/join events
-    static constexpr int kSide = 3;
-    hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork_ev[2] = {nullptr, nullptr};
-    hipEvent_t join_ev[2][kSide] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     int lds_max = 65536;
     int cu_count = 0;
     bool loaded = false;
@@ -270,28 +264,6 @@ std::vector<hipEvent_t> take_events(ggx_ctx* ctx, int count) {
     return ev;
 }
 
-// Work of phase `ph` may run on the side streams once the main stream reaches this point.
-int fork(ggx_ctx* ctx, int ph) {
-    HIPCHK(hipEventRecord(ctx->fork_ev[ph], ctx->stream));
-    for (int k = 0; k < ggx_ctx::kSide; ++k) HIPCHK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev[ph], 0));
-    return GGX_OK;
-}
-
-// The main stream continues once the side streams have finished phase `ph`.
-int join(ggx_ctx* ctx, int ph) {
-    for (int k = 0; k < ggx_ctx::kSide; ++k) {
-        HIPCHK(hipEventRecord(ctx->join_ev[ph][k], ctx->side[k]));
-        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->join_ev[ph][k], 0));
-    }
-    return GGX_OK;
-}
-
-// Stream of the q-th launch of a phase: round robin over the main and side streams.
-hipStream_t lane_stream(const ggx_ctx* ctx, int q) {
-    const int k = q % (ggx_ctx::kSide + 1);
-    return k == 0 ? ctx->stream : ctx->side[k - 1];
-}
-
 // Convolution of the grains of one work list whose boxes have side <= nb: the register-blocked
 // kernel when both of its tiles fit in LDS, else the strip kernel.
 template <class T>
@@ -336,7 +308,7 @@ int launch_step(ggx_ctx* ctx) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(32), 0, st, S);
     hipLaunchKernelGGL(k_prep, dim3((NS + 255) / 256), dim3(256), 0, st, NS, ctx->flags, ctx->box, bucket_table(),
                        ctx->blists, ctx->alist, ctx->errs, S);
-    hipLaunchKernelGGL(k_lists, dim3(cus * 8), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
+    hipLaunchKernelGGL(k_lists, dim3(cus * 4), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
                        ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
     HIPCHK(hipEventRecord(ev[1], st));
     // grid per bucket: persistent (fixed) or one workgroup per grain (counts read back)
@@ -346,17 +318,14 @@ int launch_step(ggx_ctx* ctx) {
         HIPCHK(hipStreamSynchronize(st));
         for (int b = 0; b < kNumBuckets; ++b) counts[b] = ctx->h_sched->bucket_count[b];
     }
-    // convolution, one launch per bucket, largest boxes first, spread over the main and side streams
-    int rc = fork(ctx, 0);
-    if (rc) return rc;
-    for (int b = kNumBuckets - 1, q = 0; b >= 0; --b) {
+    // convolution, one launch per bucket
+    for (int b = 0; b < kNumBuckets; ++b) {
         int grid = ctx->opt.persistent ? 0 : counts[b];
         if (!ctx->opt.persistent && grid == 0) continue;
-        rc = launch_conv<T>(ctx, lane_stream(ctx, q++), ctx->blists + (size_t)b * NS, &S->bucket_count[b],
-                            &S->bucket_cursor[b], kBucketEdges[b], grid, (const T*)P, Tt);
+        int rc = launch_conv<T>(ctx, st, ctx->blists + (size_t)b * NS, &S->bucket_count[b], &S->bucket_cursor[b],
+                                kBucketEdges[b], grid, (const T*)P, Tt);
         if (rc) return rc;
     }
-    if ((rc = join(ctx, 0))) return rc;
     HIPCHK(hipEventRecord(ev[2], st));
     // fused comparison / level set / redistancing, one persistent launch per bucket
     ContourOut O;
@@ -365,8 +334,7 @@ int launch_step(ggx_ctx* ctx) {
     O.nb_cnt = ctx->nb_cnt; O.nb_id = ctx->nb_id; O.nb_len = ctx->nb_len;
     O.box_next = ctx->box_next; O.off_next = ctx->off_next; O.flags = ctx->flags; O.err = ctx->errs;
     O.tops = &S->tops[0]; O.field_cap = ctx->field_cap; O.contour_cap = ctx->contour_cap;
-    if ((rc = fork(ctx, 1))) return rc;
-    for (int b = kNumBuckets - 1, q = 0; b >= 0; --b) {
+    for (int b = 0; b < kNumBuckets; ++b) {
         int nb = kBucketEdges[b];
         LevelLayout full = level_layout<T>(nb, true), small = level_layout<T>(nb, false);
         long long lds = full.total <= level_max ? full.total : small.total;
@@ -385,11 +353,9 @@ int launch_step(ggx_ctx* ctx) {
         int grid = cus * per_cu(ctx, lds + ctx->static_level[ti]);
         if (!ctx->opt.persistent) grid = counts[b];
         if (grid == 0) continue;
-        hipStream_t ls = lane_stream(ctx, q++);
-        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(grid), dim3(256), (size_t)lds, ls, A);
-        else hipLaunchKernelGGL((k_level<T, false>), dim3(grid), dim3(256), (size_t)lds, ls, A);
+        if (lds == full.total) hipLaunchKernelGGL((k_level<T, true>), dim3(grid), dim3(256), (size_t)lds, st, A);
+        else hipLaunchKernelGGL((k_level<T, false>), dim3(grid), dim3(256), (size_t)lds, st, A);
     }
-    if ((rc = join(ctx, 1))) return rc;
     HIPCHK(hipEventRecord(ev[3], st));
     HIPCHK(hipGetLastError());
     ctx->ev_pending.push_back(ev);
@@ -523,13 +489,6 @@ extern "C" int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx**
     hipError_t e = hipSetDevice(ctx->dev);
     if (e != hipSuccess) { delete ctx; return GGX_E_HIP; }
     if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return GGX_E_HIP; }
-    for (int k = 0; k < ggx_ctx::kSide; ++k)
-        if (hipStreamCreate(&ctx->side[k]) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
-    for (int ph = 0; ph < 2; ++ph) {
-        if (hipEventCreateWithFlags(&ctx->fork_ev[ph], hipEventDisableTiming) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
-        for (int k = 0; k < ggx_ctx::kSide; ++k)
-            if (hipEventCreateWithFlags(&ctx->join_ev[ph][k], hipEventDisableTiming) != hipSuccess) { ggx_destroy(ctx); return GGX_E_HIP; }
-    }
     if (raise_lds(ctx) != GGX_OK) { delete ctx; return GGX_E_HIP; }
     std::memset(&ctx->st, 0, sizeof(ctx->st));
     ctx->loop = s->start_time;
@@ -551,13 +510,6 @@ extern "C" void ggx_destroy(ggx_ctx* ctx) {
     if (ctx->h_sched) (void)hipHostFree(ctx->h_sched);
     for (auto& v : ctx->ev_pending) for (auto e : v) (void)hipEventDestroy(e);
     for (auto& v : ctx->ev_free) for (auto e : v) (void)hipEventDestroy(e);
-    for (int ph = 0; ph < 2; ++ph) {
-        if (ctx->fork_ev[ph]) (void)hipEventDestroy(ctx->fork_ev[ph]);
-        for (int k = 0; k < ggx_ctx::kSide; ++k)
-            if (ctx->join_ev[ph][k]) (void)hipEventDestroy(ctx->join_ev[ph][k]);
-    }
-    for (int k = 0; k < ggx_ctx::kSide; ++k)
-        if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

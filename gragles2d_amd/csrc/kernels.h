@@ -136,37 +136,55 @@ __device__ __forceinline__ int next_item(int* cursor, int* s_slot) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Comparison lists, LSbox::computeSecondOrderNeighbours (box.cpp:1027-1068).  One 64-lane wave
-// per grain, four grains per workgroup.  cmp = the existing entries of the previous list, in
-// order, then every existing neighbour-of-neighbour whose input box intersects (closed
-// intervals, checkIntersection, box.cpp:881-888), first occurrence only, excluding self and the
-// boundary grain.  Candidates are visited in (entry, entry's entry) order, 64 at a time; a
-// per-wave LDS hash maps each id to the smallest candidate index seen (entries of the first
-// part map to -1), so a candidate is a first occurrence iff the hash holds its own index, and
-// ballots keep the output in order.
+// Comparison lists, LSbox::computeSecondOrderNeighbours (box.cpp:1027-1068).  One 64-lane
+// workgroup per grain.  cmp = the existing entries of the previous list, then every existing
+// neighbour-of-neighbour whose input box intersects (closed intervals, checkIntersection,
+// box.cpp:881-888), first occurrence only, excluding self and the boundary grain.  The
+// first-occurrence test is an LDS hash (grain id -> smallest candidate index), so the
+// candidates are examined in parallel and compacted in their original order.
 // ------------------------------------------------------------------------------------------
-constexpr int kLHash = 512;   // hash slots per wave
-constexpr int kLFill = 384;   // distinct ids allowed per grain (> 2 kCmpCap: beyond it the list overflows anyway)
+constexpr int kHash = 4096;      // LDS hash slots (grain id -> smallest candidate index)
+constexpr int kHashFill = 3072;  // distinct keys allowed per grain (keeps probing bounded)
+constexpr int kCandChunk = 2048; // neighbours-of-neighbours staged in LDS at a time
 
-// Open-addressing insert (key-CAS) into one wave's table: the slot holding `key`, or -1 when
-// the table already holds kLFill keys (probing stays bounded).
+// Block-wide exclusive scan of one int per lane; returns the exclusive prefix, *total gets the sum.
+__device__ __forceinline__ int block_scan(int v, int* s_tmp, int* total) {
+    const int t = threadIdx.x;
+    s_tmp[t] = v;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+        int add = (t >= d) ? s_tmp[t - d] : 0;
+        __syncthreads();
+        s_tmp[t] += add;
+        __syncthreads();
+    }
+    int incl = s_tmp[t];
+    *total = s_tmp[blockDim.x - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Open-addressing insert (key-CAS): returns the slot holding `key`, or -1 when the table holds
+// kHashFill keys already (the caller reports an error; probing stays bounded).
 __device__ __forceinline__ int hash_insert(int* h_key, int* fill, int key) {
-    unsigned hs = ((unsigned)key * 2654435761u) & (kLHash - 1);
+    unsigned hs = ((unsigned)key * 2654435761u) & (kHash - 1);
     while (true) {
-        const int cur = h_key[hs];
+        int cur = h_key[hs];
         if (cur == key) return (int)hs;
         if (cur == -1) {
-            if (*fill >= kLFill) return -1;
-            const int old = atomicCAS(&h_key[hs], -1, key);
+            if (*fill >= kHashFill) return -1;
+            int old = atomicCAS(&h_key[hs], -1, key);
             if (old == -1) { atomicAdd(fill, 1); return (int)hs; }
             if (old == key) return (int)hs;
         }
-        hs = (hs + 1) & (kLHash - 1);
+        hs = (hs + 1) & (kHash - 1);
     }
 }
 
-__device__ __forceinline__ unsigned long long lanes_below() {
-    return (1ull << (threadIdx.x & 63)) - 1ull;
+__device__ __forceinline__ int hash_find(const int* h_key, int key) {
+    unsigned hs = ((unsigned)key * 2654435761u) & (kHash - 1);
+    while (h_key[hs] != key) hs = (hs + 1) & (kHash - 1);
+    return (int)hs;
 }
 
 __global__ void __launch_bounds__(256) k_lists(const int* __restrict__ alist, const int* __restrict__ count,
@@ -174,87 +192,105 @@ __global__ void __launch_bounds__(256) k_lists(const int* __restrict__ alist, co
                                                const int* __restrict__ flags, const int* __restrict__ sec_cnt,
                                                const int* __restrict__ sec, int* __restrict__ cmp_cnt,
                                                int* __restrict__ cmp, int* __restrict__ err) {
-    __shared__ int s_key[4][kLHash];
-    __shared__ int s_min[4][kLHash];
-    __shared__ int s_fill[4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int* hk = s_key[w];
-    int* hm = s_min[w];
+    __shared__ int h_key[kHash];
+    __shared__ int h_min[kHash];
+    __shared__ int s_cand[kCandChunk];
+    __shared__ int s_sec[kCmpCap];
+    __shared__ int s_off[kCmpCap + 1];
+    __shared__ int s_tmp[256];
+    __shared__ int s_slot, s_fill, s_bad;
+    const int t = threadIdx.x;
     while (true) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(cursor, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
+        const int item = next_item(cursor, &s_slot);
         if (item >= *count) break;
         const int g = alist[item];
         const Box bg = box[g];
         const int ns = sec_cnt[g];
         int* out = cmp + (size_t)g * kCmpCap;
-        for (int k = lane; k < kLHash; k += 64) { hk[k] = -1; hm[k] = 0x7fffffff; }
-        if (lane == 0) s_fill[w] = 0;
-        __builtin_amdgcn_wave_barrier();  // the table is this wave's alone: a wave-level fence suffices
-        bool bad = false, over = false;
-        int written = 0;
-        // part 1: existing entries of the previous list, in order (box.cpp:1038-1041)
-        for (int a0 = 0; a0 < ns; a0 += 64) {
-            const int a = a0 + lane;
-            int id = -1;
-            bool ex = false;
-            if (a < ns) {
-                id = sec[(size_t)g * kCmpCap + a];
-                ex = (flags[id] & kExists) != 0;
-            }
-            const unsigned long long m = __ballot(ex);
-            if (ex) {
-                out[written + __popcll(m & lanes_below())] = id;
-                const int hs = hash_insert(hk, &s_fill[w], id);
-                if (hs >= 0) hm[hs] = -1;
-                else bad = true;
-            }
-            written += __popcll(m);
+        for (int k = t; k < kHash; k += blockDim.x) { h_key[k] = -1; h_min[k] = 0x7fffffff; }
+        if (t == 0) { s_fill = 0; s_bad = 0; }
+        int ex = 0, len = 0;
+        if (t < ns) {
+            int nb = sec[(size_t)g * kCmpCap + t];
+            s_sec[t] = nb;
+            ex = (flags[nb] & kExists) ? 1 : 0;
+            len = sec_cnt[nb];
         }
-        __builtin_amdgcn_wave_barrier();
-        // part 2: neighbours of the entries, first occurrences in order (box.cpp:1042-1066)
-        int cbase = 0;
-        for (int a = 0; a < ns; ++a) {
-            const int sa = sec[(size_t)g * kCmpCap + a];
-            const int L = sec_cnt[sa];
-            const int* snb = sec + (size_t)sa * kCmpCap;
-            for (int q0 = 0; q0 < L; q0 += 64) {
-                const int q = q0 + lane;
-                int c = -1, hs = -1;
-                bool valid = false;
-                if (q < L) {
-                    c = snb[q];
-                    valid = (flags[c] & kExists) && c != g && c != 0;
+        int ntot;
+        int koff = block_scan(len, s_tmp, &ntot);   // candidate offset of entry t
+        if (t < ns) s_off[t] = koff;
+        if (t == 0) s_off[ns] = ntot;
+        int n1;
+        int pos1 = block_scan(ex, s_tmp, &n1);      // rank among existing entries
+        // part 1: existing entries of the previous list, in order (box.cpp:1038-1041)
+        if (t < ns && ex) {
+            out[pos1] = s_sec[t];
+            int hs = hash_insert(h_key, &s_fill, s_sec[t]);
+            if (hs >= 0) h_min[hs] = -1;  // already in the list
+            else s_bad = 1;
+        }
+        __syncthreads();
+        // candidates, flattened in (entry, entry's entry) order (box.cpp:1042-1048), staged in
+        // chunks; pass 0 records each id's first valid index, pass 1 compacts first occurrences
+        int written = n1;
+        bool over = false;
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int c0 = 0; c0 < ntot; c0 += kCandChunk) {
+                const int cl = min(kCandChunk, ntot - c0);
+                for (int a = 0; a < ns; ++a) {
+                    const int lo = max(s_off[a], c0), hi = min(s_off[a + 1], c0 + cl);
+                    const int* snb = sec + (size_t)s_sec[a] * kCmpCap - s_off[a];
+                    for (int k = lo + t; k < hi; k += blockDim.x) s_cand[k - c0] = snb[k];
+                }
+                __syncthreads();
+                for (int q = t; q < cl; q += blockDim.x) {
+                    int c = s_cand[q];
+                    bool valid = (flags[c] & kExists) && c != g && c != 0;
                     if (valid) {
-                        const Box bc = box[c];
+                        Box bc = box[c];
                         valid = !(bg.x0 > bc.x0 + bc.n || bg.x0 + bg.n < bc.x0 || bg.y0 > bc.y0 + bc.n ||
                                   bg.y0 + bg.n < bc.y0);
                     }
+                    if (!valid) { s_cand[q] = -1; continue; }
+                    if (pass == 0) {
+                        int hs = hash_insert(h_key, &s_fill, c);
+                        if (hs >= 0) atomicMin(&h_min[hs], c0 + q);  // stays -1 for part-1 entries
+                        else s_bad = 1;
+                    }
                 }
-                if (valid) {
-                    hs = hash_insert(hk, &s_fill[w], c);
-                    if (hs >= 0) atomicMin(&hm[hs], cbase + q);
-                    else bad = true;
+                __syncthreads();
+                if (pass == 1 && !s_bad) {
+                    // ordered compaction of first occurrences (box.cpp:1050-1066)
+                    const int per = (cl + (int)blockDim.x - 1) / (int)blockDim.x;
+                    const int q0 = t * per, q1 = min(cl, q0 + per);
+                    int keep = 0;
+                    for (int q = q0; q < q1; ++q) {
+                        int c = s_cand[q];
+                        if (c >= 0 && h_min[hash_find(h_key, c)] == c0 + q) keep++;
+                    }
+                    int nk;
+                    int w = written + block_scan(keep, s_tmp, &nk);
+                    for (int q = q0; q < q1; ++q) {
+                        int c = s_cand[q];
+                        if (c >= 0 && h_min[hash_find(h_key, c)] == c0 + q) {
+                            if (w < kCmpCap) out[w] = c; else over = true;
+                            w++;
+                        }
+                    }
+                    written += nk;
                 }
-                const bool first = valid && hs >= 0 && hm[hs] == cbase + q;
-                const unsigned long long m = __ballot(first);
-                if (first) {
-                    const int pos = written + __popcll(m & lanes_below());
-                    if (pos < kCmpCap) out[pos] = c;
-                    else over = true;
-                }
-                written += __popcll(m);
+                __syncthreads();
             }
-            cbase += L;
         }
-        if (__ballot(bad || over) != 0ull || written > kCmpCap) {
-            if (lane == 0) err[g] = kErrCmpCap;
+        if (over) err[g] = kErrCmpCap;
+        if (t == 0) {
+            if (written > kCmpCap || s_bad) err[g] = kErrCmpCap;
+            cmp_cnt[g] = min(written, kCmpCap);
         }
-        if (lane == 0) cmp_cnt[g] = min(written, kCmpCap);
     }
 }
 
+// ------------------------------------------------------------------------------------------
 // Circulant tables: the table of box side n holds K_n[(t - kKPad) mod n] for
 // t in [0, 2n + 2 kKPad), so that every index a register block needs is in range without a
 // modulo.

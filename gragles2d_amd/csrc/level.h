@@ -469,64 +469,37 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const double band = 0.7 * G.delta;
             int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
             unsigned hits = 0;
-            // Each lane keeps kPx pixels of a chunk in registers.  Pass 1 (ALU only) records, per
-            // pixel, the list positions of the first kHit neighbours whose boxes cover it; pass 2
-            // issues all their gathers at once and takes the maximum in list order (strict >,
-            // first wins); pixels covered by more neighbours finish the list one by one.
-            constexpr int kPx = 4, kHit = 4;
+            // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
+            // chunk, so the gathers of one neighbour are independent loads issued together;
+            // per pixel the neighbours are still met in list order (strict >, first wins).
+            constexpr int kPx = 8;
             for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
-                int yy[kPx], xx[kPx], id[kPx], nh[kPx];
-                uint32_t hl[kPx];
+                int yy[kPx], xx[kPx], id[kPx];
                 T mx[kPx];
 #pragma unroll
                 for (int t = 0; t < kPx; ++t) {
                     const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
-                    yy[t] = p < nn ? (int)(p / n) : -(1 << 20);  // a row no box covers
+                    yy[t] = p < nn ? (int)(p / n) : n;  // row n lies outside every overlap
                     xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
                     mx[t] = T(-1.0);
                     id[t] = -1;
-                    nh[t] = 0;
-                    hl[t] = 0u;
                 }
-                const int r0 = (int)(c0 / n), r1 = (int)min((long long)n - 1, (c0 + (long long)kPx * blockDim.x - 1) / n);
+#pragma unroll 2
                 for (int k = 0; k < cnt; ++k) {
                     const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
-                    if (dy > r1 || dy + nk <= r0) continue;  // the neighbour misses every row of the chunk
+                    const T* nsrc = A.tilde + s_off[k];
+                    T v[kPx];
+                    bool in[kPx];
 #pragma unroll
                     for (int t = 0; t < kPx; ++t) {
-                        const bool in = (unsigned)(yy[t] - dy) < (unsigned)nk && (unsigned)(xx[t] - dx) < (unsigned)nk;
-                        if (in) {
-                            if (nh[t] < kHit) hl[t] |= (uint32_t)k << (8 * nh[t]);
-                            nh[t]++;
-                        }
+                        const int ly = yy[t] - dy, lx = xx[t] - dx;
+                        in[t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
+                        v[t] = in[t] ? nsrc[(long long)ly * nk + lx] : T(0);
                     }
-                }
-                T v[kHit][kPx];
-#pragma unroll
-                for (int hh = 0; hh < kHit; ++hh)
 #pragma unroll
                     for (int t = 0; t < kPx; ++t) {
-                        v[hh][t] = T(0);
-                        if (hh < nh[t]) {
-                            const int k = (int)((hl[t] >> (8 * hh)) & 255u);
-                            const int nk = s_n[k];
-                            v[hh][t] = A.tilde[s_off[k] + (long long)(yy[t] + b.y0 - s_y0[k]) * nk + (xx[t] + b.x0 - s_x0[k])];
-                        }
-                    }
-#pragma unroll
-                for (int t = 0; t < kPx; ++t) {
-                    hits += (unsigned)nh[t];
-#pragma unroll
-                    for (int hh = 0; hh < kHit; ++hh)
-                        if (hh < nh[t] && v[hh][t] > mx[t]) { mx[t] = v[hh][t]; id[t] = s_id[(hl[t] >> (8 * hh)) & 255u]; }
-                    if (nh[t] > kHit) {  // rare: the remaining covering neighbours in list order
-                        for (int k = (int)(hl[t] >> 24) + 1; k < cnt; ++k) {
-                            const int ly = yy[t] + b.y0 - s_y0[k], lx = xx[t] + b.x0 - s_x0[k], nk = s_n[k];
-                            if ((unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk) {
-                                const T w = A.tilde[s_off[k] + (long long)ly * nk + lx];
-                                if (w > mx[t]) { mx[t] = w; id[t] = s_id[k]; }
-                            }
-                        }
+                        hits += in[t] ? 1u : 0u;
+                        if (in[t] && v[t] > mx[t]) { mx[t] = v[t]; id[t] = s_id[k]; }
                     }
                 }
 #pragma unroll
