@@ -132,30 +132,6 @@ __device__ __forceinline__ int walk_squares(int sx, int sy, int x0, int y0, int 
     return st;
 }
 
-// Inside bits (value >= 0) of a box up to 128 x 128 in the VGPRs of one wave: lane l holds
-// rows l and l + 64, each as two 64-column words, built with one ballot per row and word.
-// The walk then runs wave-uniformly in scalar registers, fetching rows with readlane.
-struct WaveMask {
-    unsigned long long m00, m01, m10, m11;  // [row >= 64][column >= 64]
-};
-
-template <class T>
-__device__ __forceinline__ WaveMask wave_masks(const T* C, int n, int cp) {
-    const int lane = threadIdx.x & 63;
-    WaveMask M{0ull, 0ull, 0ull, 0ull};
-#pragma unroll 4
-    for (int r = 0; r < n; ++r) {
-        const T* row = C + (long long)r * cp;
-        const unsigned long long w0 = __ballot(lane < n && row[lane] >= T(0));
-        const unsigned long long w1 = n > 64 ? __ballot(64 + lane < n && row[64 + lane] >= T(0)) : 0ull;
-        if (lane == (r & 63)) {
-            if (r < 64) { M.m00 = w0; M.m01 = w1; }
-            else { M.m10 = w0; M.m11 = w1; }
-        }
-    }
-    return M;
-}
-
 __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int src) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, src);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), src);
@@ -199,57 +175,90 @@ __device__ __forceinline__ int block_flag_scan(bool f, int* s_wt, int* total) {
     return base + pre;
 }
 
-// Wave 0 of the workgroup: start pixel (first inside pixel in row-major order) and walk.
-// Writes s_int[0] (start index or INT_MAX when nothing is inside), s_int[1] (moves) and, on
-// failure, s_int[3].
-__device__ __forceinline__ void wave_trace(const WaveMask& M, int n, int x0, int y0, int ccap, uint32_t* moves,
-                                           int* s_int) {
+// Marching-squares case table of a box, one byte per square: the square (r, c), r, c in
+// [0, n], has its top-left corner at pixel (y0 - 1 + r, x0 - 1 + c).  Bits 0-1: the direction
+// of an ordinary case, SQ_GEN: the case generates a point, SQ_S6 / SQ_S9: the two saddles
+// (resolved by the previous direction), SQ_NONE: no boundary (the walk keeps its direction).
+// The cases and directions are those of marchingSquares.cpp:44-176.
+enum { SQ_GEN = 4, SQ_S6 = 8, SQ_S9 = 16, SQ_NONE = 32 };
+constexpr int kTabMax = 128;  // largest box side walked through the table
+
+__device__ __forceinline__ unsigned char square_code(int mask) {
+    switch (mask) {
+        case MS_TL: return D_UP | SQ_GEN;
+        case MS_TR: return D_RIGHT | SQ_GEN;
+        case MS_TL | MS_TR: return D_RIGHT | SQ_GEN;
+        case MS_BL: return D_LEFT | SQ_GEN;
+        case MS_BL | MS_TL: return D_UP | SQ_GEN;
+        case MS_BL | MS_TR: return SQ_S6;
+        case MS_BL | MS_TR | MS_TL: return D_RIGHT | SQ_GEN;
+        case MS_BR: return D_DOWN | SQ_GEN;
+        case MS_BR | MS_TL: return SQ_S9;
+        case MS_BR | MS_TR: return D_DOWN | SQ_GEN;
+        case MS_BR | MS_TR | MS_TL: return D_DOWN | SQ_GEN;
+        case MS_BR | MS_BL: return D_LEFT | SQ_GEN;
+        case MS_BR | MS_BL | MS_TL: return D_UP | SQ_GEN;
+        case MS_BR | MS_BL | MS_TR: return D_LEFT | SQ_GEN;
+        default: return SQ_NONE;
+    }
+}
+
+// All lanes: the case of every square (inside = value >= 0 within the box).
+template <class T>
+__device__ __forceinline__ void build_squares(const T* C, int n, int cp, unsigned char* tab) {
+    const int w = n + 1;
+    for (int q = threadIdx.x; q < w * w; q += blockDim.x) {
+        const int r = q / w, c = q - r * w;
+        auto in = [&](int pr, int pc) -> bool {
+            return (unsigned)pr < (unsigned)n && (unsigned)pc < (unsigned)n && C[pr * cp + pc] >= T(0);
+        };
+        const int mask = (in(r - 1, c - 1) ? MS_TL : 0) | (in(r - 1, c) ? MS_TR : 0) | (in(r, c - 1) ? MS_BL : 0) |
+                         (in(r, c) ? MS_BR : 0);
+        tab[q] = square_code(mask);
+    }
+}
+
+// One wave walks the squares from the start pixel (sr, sc) (box-local; its square is (sr, sc)),
+// wave-uniformly: one LDS byte per move.  Moves are recorded as walk_squares records them.
+// Returns 0, kErrContourCap or kErrTrace.
+__device__ __forceinline__ int table_walk(const unsigned char* tab, int n, int sr, int sc, int ccap, uint32_t* moves,
+                                          int* nmoves) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long any0 = __ballot((M.m00 | M.m01) != 0ull);
-    const unsigned long long any1 = __ballot((M.m10 | M.m11) != 0ull);
-    int r;
-    if (any0) r = __builtin_ctzll(any0);
-    else if (any1) r = 64 + __builtin_ctzll(any1);
-    else {
-        if (lane == 0) s_int[0] = 0x7fffffff;
-        return;
-    }
-    // the four words stay in named registers; every choice among them is a uniform branch
-    // around a readlane, so no select between their addresses can send them to scratch
-    const unsigned long long m00 = M.m00, m01 = M.m01, m10 = M.m10, m11 = M.m11;
-    unsigned long long a, b;
-    if (r < 64) { a = readlane64(m00, r); b = readlane64(m01, r); }
-    else { a = readlane64(m10, r - 64); b = readlane64(m11, r - 64); }
-    const int c = a ? __builtin_ctzll(a) : 64 + __builtin_ctzll(b);
-    auto inside = [=](int row, int col) -> bool {
-        const int rr = row - y0, q = col - x0;
-        if ((unsigned)rr >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
-        unsigned long long w;
-        if (rr < 64) {
-            if (q < 64) w = readlane64(m00, rr);
-            else w = readlane64(m01, rr);
+    const int w = n + 1;
+    const long long itcap = 8LL * (n + 2) * (n + 2) + 16;
+    long long it = 0;
+    int r = sr, c = sc, next = D_NONE, prev = D_NONE, nm = 0, st = 0;
+    do {
+        int e = SQ_NONE;
+        if ((unsigned)r < (unsigned)w && (unsigned)c < (unsigned)w) e = __builtin_amdgcn_readfirstlane((int)tab[r * w + c]);
+        bool gen = false;
+        if (e & SQ_NONE) {
+        } else if (e & SQ_S6) {
+            next = (prev == D_UP) ? D_RIGHT : D_LEFT;
+        } else if (e & SQ_S9) {
+            next = (prev == D_RIGHT) ? D_DOWN : D_UP;
         } else {
-            if (q < 64) w = readlane64(m10, rr - 64);
-            else w = readlane64(m11, rr - 64);
+            next = e & 3;
+            gen = true;
         }
-        return (w >> (q & 63)) & 1ull;
-    };
-    auto emit = [&](int k, uint32_t mv) {
-        if (lane == 0) moves[k] = mv;
-    };
-    int nm = 0;
-    const int st = walk_squares(x0 + c, y0 + r, x0, y0, n, ccap, inside, emit, &nm);
-    if (lane == 0) {
-        s_int[0] = r * n + c;
-        s_int[1] = nm;
-        if (st) s_int[3] = st;
-    }
+        if (gen) {
+            if (nm >= ccap) { st = kErrContourCap; break; }
+            if (lane == 0) moves[nm] = (uint32_t)next | ((uint32_t)c << 2) | ((uint32_t)r << 16);
+            nm++;
+        }
+        r += (next == D_DOWN) - (next == D_UP);
+        c += (next == D_RIGHT) - (next == D_LEFT);
+        prev = next;
+        if (++it > itcap) { st = kErrTrace; break; }
+    } while (c != sc || r != sr);
+    *nmoves = nm;
+    return st;
 }
 
 // LDS layout of the fused kernel for boxes up to side nb.  Full layout: the C tile, then the O
 // tile, which until redistancing hosts the per-point work arrays (36 bytes per contour point:
 // the compacted contour; the interpolated moves, later the segment lengths and shoelace
-// terms; the recorded moves, later the neighbour probes).  Small layout (boxes too large for
+// terms; the recorded moves, later the neighbour probes) and the square case table.  Small layout (boxes too large for
 // LDS): C and O stay in global memory, only the work arrays are in LDS.
 struct LevelLayout {
     long long ctile, otile, total;
@@ -260,12 +269,71 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     LevelLayout L;
     const long long ccap = contour_cap(nb);
     const long long no = nb + 8;
-    const long long work = (ccap * 36 + 15) & ~15LL;
+    const long long tab = nb <= kTabMax ? (long long)(nb + 1) * (nb + 1) : 0;
+    const long long work = (ccap * 36 + tab + 15) & ~15LL;
     L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
     L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
     if (work > L.otile) L.otile = work;
     L.total = L.ctile + L.otile;
     return L;
+}
+
+// Interface crossings of one sweep line, found before the sequential sweep so that the two
+// divisions of each crossing (slope = (a1 - a) / h, d0 = -a / slope, box.cpp:1301-1306) run in a
+// few wave-wide rounds instead of inside the recurrence, where one crossing lane would make
+// the whole wave wait on them.  Up to four crossings per line are kept (in order) in a shift
+// register; further ones are computed in the sweep itself.
+struct Crossings {
+    int i0, i1, i2, i3, cnt;
+    double d0, d1, d2, d3, s0, s1, s2, s3;
+    __device__ __forceinline__ void pop() {
+        i0 = i1; i1 = i2; i2 = i3; i3 = 0x7fffffff;
+        d0 = d1; d1 = d2; d2 = d3;
+        s0 = s1; s1 = s2; s2 = s3;
+    }
+};
+
+// value(k) gives the compared field at step k of the line; steps k in [k0, k1) may cross
+// (value(k) * value(k + 1) <= 0).  Every lane of the wave must call it.
+template <class V>
+__device__ __forceinline__ Crossings find_crossings(V value, int k0, int k1, bool active, double h) {
+    Crossings X;
+    X.i0 = X.i1 = X.i2 = X.i3 = 0x7fffffff;
+    X.cnt = 0;
+    X.d0 = X.d1 = X.d2 = X.d3 = 0.0;
+    X.s0 = X.s1 = X.s2 = X.s3 = 0.0;
+    if (active && k0 < k1) {
+        double a = value(k0);
+        for (int k = k0; k < k1; ++k) {
+            const double a1 = value(k + 1);
+            if (a * a1 <= 0.0) {
+                if (X.cnt == 0) X.i0 = k;
+                else if (X.cnt == 1) X.i1 = k;
+                else if (X.cnt == 2) X.i2 = k;
+                else if (X.cnt == 3) X.i3 = k;
+                X.cnt++;
+            }
+            a = a1;
+        }
+    }
+    // the divisions, one wave-wide round per slot
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int iq = q == 0 ? X.i0 : q == 1 ? X.i1 : q == 2 ? X.i2 : X.i3;
+        const bool has = iq != 0x7fffffff;
+        if (__ballot(has) == 0ull) break;
+        if (has) {
+            const double a = value(iq), a1 = value(iq + 1);
+            const double slope = (a1 - a) / h;
+            const double d = -a / slope;
+            const double sg = (double)sgn_of(slope);
+            if (q == 0) { X.d0 = d; X.s0 = sg; }
+            else if (q == 1) { X.d1 = d; X.s1 = sg; }
+            else if (q == 2) { X.d2 = d; X.s2 = sg; }
+            else { X.d3 = d; X.s3 = sg; }
+        }
+    }
+    return X;
 }
 
 // LSbox::executeRedistancing (box.cpp:1259-1399) of one grain: Ob is the output tile (LDS with
@@ -292,31 +360,38 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, int cp,
     }
     __syncthreads();
     // y forward (box.cpp:1291-1319)
-    for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
-        if (iymin >= oymax - 1) break;
+    for (int j0 = ixmin; j0 < oxmax; j0 += blockDim.x) {
+        const int j = j0 + (int)threadIdx.x;
+        const bool on = j < oxmax && iymin < oymax - 1;
         const bool colin = j < ixmax;
-        T cur = OV(iymin, j);
-        T o1 = OV(iymin + 1, j);
-        double a = (colin && iymin < iymax) ? (double)CI(iymin, j) : 0.0;
-        double a1 = (colin && iymin + 1 < iymax) ? (double)CI(iymin + 1, j) : 0.0;
-        for (int i = iymin; i < oymax - 1; i++) {
-            const T o2 = (i + 2 < oymax) ? OV(i + 2, j) : T(0);
-            const double a2 = (colin && i + 2 < iymax) ? (double)CI(i + 2, j) : 0.0;
-            T nxt = o1;
-            double cand;
-            if (colin && i < iymax - 1) {
-                if (a * a1 <= 0.0) {
-                    double slope = (a1 - a) / h;
-                    double d0 = -a / slope;
-                    if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
+        // crossings need i < iymax - 1 (both values inside the old box)
+        Crossings X = find_crossings([&](int i) { return (double)CI(i, j); }, iymin, min(oymax - 1, iymax - 1),
+                                     on && colin, h);
+        if (on) {
+            T cur = OV(iymin, j);
+            T o1 = OV(iymin + 1, j);
+            double a = (colin && iymin < iymax) ? (double)CI(iymin, j) : 0.0;
+            double a1 = (colin && iymin + 1 < iymax) ? (double)CI(iymin + 1, j) : 0.0;
+            for (int i = iymin; i < oymax - 1; i++) {
+                const T o2 = (i + 2 < oymax) ? OV(i + 2, j) : T(0);
+                const double a2 = (colin && i + 2 < iymax) ? (double)CI(i + 2, j) : 0.0;
+                T nxt = o1;
+                double cand;
+                if (colin && i < iymax - 1) {
+                    if (a * a1 <= 0.0) {
+                        double d0, sg;
+                        if (X.i0 == i) { d0 = X.d0; sg = X.s0; X.pop(); }
+                        else { const double slope = (a1 - a) / h; d0 = -a / slope; sg = (double)sgn_of(slope); }
+                        if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sg); OV(i, j) = cur; }
+                    }
+                    cand = (double)cur + (sgn_of(a1) * h);
+                } else {
+                    cand = (double)cur + (sgn_of(nxt) * h);
                 }
-                cand = (double)cur + (sgn_of(a1) * h);
-            } else {
-                cand = (double)cur + (sgn_of(nxt) * h);
+                if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i + 1, j) = nxt; }
+                cur = nxt;
+                a = a1; a1 = a2; o1 = o2;
             }
-            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i + 1, j) = nxt; }
-            cur = nxt;
-            a = a1; a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
@@ -341,31 +416,37 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, int cp,
     }
     __syncthreads();
     // x forward (1339-1367)
-    for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
-        if (ixmin >= oxmax - 1) break;
+    for (int i0 = iymin; i0 < oymax; i0 += blockDim.x) {
+        const int i = i0 + (int)threadIdx.x;
+        const bool on = i < oymax && ixmin < oxmax - 1;
         const bool rowin = i < iymax;
-        T cur = OV(i, ixmin);
-        T o1 = OV(i, ixmin + 1);
-        double a = (rowin && ixmin < ixmax) ? (double)CI(i, ixmin) : 0.0;
-        double a1 = (rowin && ixmin + 1 < ixmax) ? (double)CI(i, ixmin + 1) : 0.0;
-        for (int j = ixmin; j < oxmax - 1; j++) {
-            const T o2 = (j + 2 < oxmax) ? OV(i, j + 2) : T(0);
-            const double a2 = (rowin && j + 2 < ixmax) ? (double)CI(i, j + 2) : 0.0;
-            T nxt = o1;
-            double cand;
-            if (j < ixmax - 1 && rowin) {
-                if (a * a1 <= 0.0) {
-                    double slope = (a1 - a) / h;
-                    double d0 = -a / slope;
-                    if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sgn_of(slope)); OV(i, j) = cur; }
+        Crossings X = find_crossings([&](int j) { return (double)CI(i, j); }, ixmin, min(oxmax - 1, ixmax - 1),
+                                     on && rowin, h);
+        if (on) {
+            T cur = OV(i, ixmin);
+            T o1 = OV(i, ixmin + 1);
+            double a = (rowin && ixmin < ixmax) ? (double)CI(i, ixmin) : 0.0;
+            double a1 = (rowin && ixmin + 1 < ixmax) ? (double)CI(i, ixmin + 1) : 0.0;
+            for (int j = ixmin; j < oxmax - 1; j++) {
+                const T o2 = (j + 2 < oxmax) ? OV(i, j + 2) : T(0);
+                const double a2 = (rowin && j + 2 < ixmax) ? (double)CI(i, j + 2) : 0.0;
+                T nxt = o1;
+                double cand;
+                if (j < ixmax - 1 && rowin) {
+                    if (a * a1 <= 0.0) {
+                        double d0, sg;
+                        if (X.i0 == j) { d0 = X.d0; sg = X.s0; X.pop(); }
+                        else { const double slope = (a1 - a) / h; d0 = -a / slope; sg = (double)sgn_of(slope); }
+                        if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sg); OV(i, j) = cur; }
+                    }
+                    cand = (double)cur + (sgn_of(a1) * h);
+                } else {
+                    cand = (double)cur + (sgn_of(nxt) * h);
                 }
-                cand = (double)cur + (sgn_of(a1) * h);
-            } else {
-                cand = (double)cur + (sgn_of(nxt) * h);
+                if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j + 1) = nxt; }
+                cur = nxt;
+                a = a1; a1 = a2; o1 = o2;
             }
-            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j + 1) = nxt; }
-            cur = nxt;
-            a = a1; a1 = a2; o1 = o2;
         }
     }
     __syncthreads();
@@ -418,6 +499,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
     double* s_term = s_len + ccap;
     int* s_cand = (int*)(s_raw + ccap);
     uint32_t* moves = (uint32_t*)s_cand;    // moves are dead before the probes are written
+    unsigned char* s_tab = (unsigned char*)(s_cand + ccap);  // square cases (boxes up to kTabMax)
 
     __shared__ int s_bb[4];
     __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len
@@ -564,18 +646,29 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
         // ---------------- contour with motion check and retries ----------------
         int retries = 0;
         while (true) {
-            if (n <= 128) {
-                if (threadIdx.x < 64) wave_trace(wave_masks(C, n, cp), n, b.x0, b.y0, ccap, moves, s_int);
-                __syncthreads();
-            } else {
-                if (threadIdx.x == 0) s_int[0] = 0x7fffffff;
-                __syncthreads();
+            if (threadIdx.x == 0) s_int[0] = 0x7fffffff;
+            __syncthreads();
+            {  // start: the first inside pixel in row-major order (marchingSquares.cpp:32-42)
                 int loc = 0x7fffffff;
                 for (long long p = threadIdx.x; p < nn; p += blockDim.x)
                     if (C[(p / n) * cp + p % n] >= T(0)) { loc = (int)p; break; }
                 if (loc != 0x7fffffff) atomicMin(&s_int[0], loc);
-                __syncthreads();
-                if (threadIdx.x == 0 && s_int[0] != 0x7fffffff) {  // lane 0 walks the squares
+            }
+            const bool tabled = nb <= kTabMax;  // the layout holds a table for buckets up to kTabMax
+            if (tabled) build_squares(C, n, cp, s_tab);
+            __syncthreads();
+            if (s_int[0] != 0x7fffffff) {
+                if (tabled) {
+                    if (threadIdx.x < 64) {
+                        const int start = __builtin_amdgcn_readfirstlane(s_int[0]);
+                        int nm = 0;
+                        const int st = table_walk(s_tab, n, start / n, start % n, ccap, moves, &nm);
+                        if (threadIdx.x == 0) {
+                            s_int[1] = nm;
+                            if (st) s_int[3] = st;
+                        }
+                    }
+                } else if (threadIdx.x == 0) {  // lane 0 walks the squares over C
                     auto inside = [&](int row, int col) -> bool {
                         int r = row - b.y0, q = col - b.x0;
                         if ((unsigned)r >= (unsigned)n || (unsigned)q >= (unsigned)n) return false;
