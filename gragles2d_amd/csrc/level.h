@@ -278,62 +278,82 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     return L;
 }
 
-// Interface crossings of one sweep line, found before the sequential sweep so that the two
-// divisions of each crossing (slope = (a1 - a) / h, d0 = -a / slope, box.cpp:1301-1306) run in a
-// few wave-wide rounds instead of inside the recurrence, where one crossing lane would make
-// the whole wave wait on them.  Up to four crossings per line are kept (in order) in a shift
-// register; further ones are computed in the sweep itself.
-struct Crossings {
-    int i0, i1, i2, i3, cnt;
-    double d0, d1, d2, d3, s0, s1, s2, s3;
-    __device__ __forceinline__ void pop() {
-        i0 = i1; i1 = i2; i2 = i3; i3 = 0x7fffffff;
-        d0 = d1; d1 = d2; d2 = d3;
-        s0 = s1; s1 = s2; s2 = s3;
-    }
-};
-
-// value(k) gives the compared field at step k of the line; steps k in [k0, k1) may cross
-// (value(k) * value(k + 1) <= 0).  Every lane of the wave must call it.
-template <class V>
-__device__ __forceinline__ Crossings find_crossings(V value, int k0, int k1, bool active, double h) {
-    Crossings X;
-    X.i0 = X.i1 = X.i2 = X.i3 = 0x7fffffff;
-    X.cnt = 0;
-    X.d0 = X.d1 = X.d2 = X.d3 = 0.0;
-    X.s0 = X.s1 = X.s2 = X.s3 = 0.0;
-    if (active && k0 < k1) {
-        double a = value(k0);
-        for (int k = k0; k < k1; ++k) {
-            const double a1 = value(k + 1);
-            if (a * a1 <= 0.0) {
-                if (X.cnt == 0) X.i0 = k;
-                else if (X.cnt == 1) X.i1 = k;
-                else if (X.cnt == 2) X.i2 = k;
-                else if (X.cnt == 3) X.i3 = k;
-                X.cnt++;
-            }
-            a = a1;
-        }
-    }
-    // the divisions, one wave-wide round per slot
+// One redistancing sweep along a line of cells t = 0..L in sweep order (box.cpp:1291-1388):
+// step t carries cur (cell t) into nxt (cell t + 1); while t < tIn both cells lie in the old box
+// and the candidate takes the sign of the compared field at cell t + 1, else the sign of nxt.
+// Forward sweeps (CROSS) first move cur onto an interface crossing between cells t and t + 1.
+// oload / ostore / cload address the line's output and compared field.  The line is processed
+// in chunks of kCh steps whose inputs are loaded together; the divisions of the crossings of a
+// chunk run in wave-wide rounds (one per crossing of the busiest lane), not at every step a lane
+// crosses.  Every lane of the wave calls this with the same L and tIn.
+template <class T, bool CROSS, class OL, class OS, class CL>
+__device__ __forceinline__ void sweep_line(OL oload, OS ostore, CL cload, int L, int tIn, bool active, bool cvalid,
+                                           double h) {
+    constexpr int kCh = 4;
+    if (L <= 0) return;
+    const int tI = cvalid ? tIn : 0;
+    T cur = active ? oload(0) : T(0);
+    double cprev = (CROSS && active && tI > 0) ? (double)cload(0) : 0.0;
+    for (int tb = 0; tb < L; tb += kCh) {
+        T ov[kCh];
+        double cv[kCh];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int iq = q == 0 ? X.i0 : q == 1 ? X.i1 : q == 2 ? X.i2 : X.i3;
-        const bool has = iq != 0x7fffffff;
-        if (__ballot(has) == 0ull) break;
-        if (has) {
-            const double a = value(iq), a1 = value(iq + 1);
-            const double slope = (a1 - a) / h;
-            const double d = -a / slope;
-            const double sg = (double)sgn_of(slope);
-            if (q == 0) { X.d0 = d; X.s0 = sg; }
-            else if (q == 1) { X.d1 = d; X.s1 = sg; }
-            else if (q == 2) { X.d2 = d; X.s2 = sg; }
-            else { X.d3 = d; X.s3 = sg; }
+        for (int q = 0; q < kCh; ++q) {
+            const int t1 = tb + 1 + q;
+            ov[q] = (active && t1 <= L) ? oload(t1) : T(0);
+            cv[q] = (active && t1 <= tI) ? (double)cload(t1) : 0.0;
         }
+        double dd[kCh], ss[kCh];
+        if constexpr (CROSS) {
+            unsigned cm = 0u;
+#pragma unroll
+            for (int q = 0; q < kCh; ++q) {
+                const int t = tb + q;
+                const double a = q == 0 ? cprev : cv[q - 1];
+                if (active && t < L && t < tI && a * cv[q] <= 0.0) cm |= 1u << q;
+                dd[q] = 0.0;
+                ss[q] = 0.0;
+            }
+            while (__ballot(cm != 0u) != 0ull) {
+                if (cm != 0u) {
+                    const int q = __builtin_ctz(cm);
+                    double a = cprev, a1 = cv[0];
+#pragma unroll
+                    for (int r = 1; r < kCh; ++r)
+                        if (r == q) { a = cv[r - 1]; a1 = cv[r]; }
+                    const double slope = (a1 - a) / h;
+                    const double d0 = -a / slope;
+                    const double sg = (double)sgn_of(slope);
+#pragma unroll
+                    for (int r = 0; r < kCh; ++r)
+                        if (r == q) { dd[r] = d0; ss[r] = sg; }
+                    cm &= cm - 1u;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kCh; ++q) {
+            const int t = tb + q;
+            if (active && t < L) {
+                T nxt = ov[q];
+                double cand;
+                if (t < tI) {
+                    if constexpr (CROSS) {
+                        const double a = q == 0 ? cprev : cv[q - 1];
+                        if (a * cv[q] <= 0.0) {
+                            if (fabs((double)cur) > fabs(dd[q])) { cur = (T)(-dd[q] * ss[q]); ostore(t, cur); }
+                        }
+                    }
+                    cand = (double)cur + (sgn_of(cv[q]) * h);
+                } else {
+                    cand = (double)cur + (sgn_of(nxt) * h);
+                }
+                if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; ostore(t + 1, nxt); }
+                cur = nxt;
+            }
+        }
+        if constexpr (CROSS) cprev = cv[kCh - 1];
     }
-    return X;
 }
 
 // LSbox::executeRedistancing (box.cpp:1259-1399) of one grain: Ob is the output tile (LDS with
@@ -359,115 +379,34 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, int cp,
         Ob[yy * pitch + xx] = T(-1.0);
     }
     __syncthreads();
-    // y forward (box.cpp:1291-1319)
+    // y forward (box.cpp:1291-1319), y backward (1321-1338): lanes own columns
     for (int j0 = ixmin; j0 < oxmax; j0 += blockDim.x) {
         const int j = j0 + (int)threadIdx.x;
-        const bool on = j < oxmax && iymin < oymax - 1;
-        const bool colin = j < ixmax;
-        // crossings need i < iymax - 1 (both values inside the old box)
-        Crossings X = find_crossings([&](int i) { return (double)CI(i, j); }, iymin, min(oymax - 1, iymax - 1),
-                                     on && colin, h);
-        if (on) {
-            T cur = OV(iymin, j);
-            T o1 = OV(iymin + 1, j);
-            double a = (colin && iymin < iymax) ? (double)CI(iymin, j) : 0.0;
-            double a1 = (colin && iymin + 1 < iymax) ? (double)CI(iymin + 1, j) : 0.0;
-            for (int i = iymin; i < oymax - 1; i++) {
-                const T o2 = (i + 2 < oymax) ? OV(i + 2, j) : T(0);
-                const double a2 = (colin && i + 2 < iymax) ? (double)CI(i + 2, j) : 0.0;
-                T nxt = o1;
-                double cand;
-                if (colin && i < iymax - 1) {
-                    if (a * a1 <= 0.0) {
-                        double d0, sg;
-                        if (X.i0 == i) { d0 = X.d0; sg = X.s0; X.pop(); }
-                        else { const double slope = (a1 - a) / h; d0 = -a / slope; sg = (double)sgn_of(slope); }
-                        if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sg); OV(i, j) = cur; }
-                    }
-                    cand = (double)cur + (sgn_of(a1) * h);
-                } else {
-                    cand = (double)cur + (sgn_of(nxt) * h);
-                }
-                if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i + 1, j) = nxt; }
-                cur = nxt;
-                a = a1; a1 = a2; o1 = o2;
-            }
-        }
+        const bool act = j < oxmax, colin = j < ixmax;
+        sweep_line<T, true>([&](int t) { return OV(iymin + t, j); }, [&](int t, T v) { OV(iymin + t, j) = v; },
+                            [&](int t) { return CI(iymin + t, j); }, oymax - 1 - iymin, iymax - 1 - iymin, act, colin, h);
     }
     __syncthreads();
-    // y backward (1321-1338)
-    for (int j = ixmin + threadIdx.x; j < oxmax; j += blockDim.x) {
-        if (iymax - 1 <= oymin) break;
-        const bool colin = j < ixmax;
-        T cur = OV(iymax - 1, j);
-        T o1 = OV(iymax - 2, j);
-        double a1 = (colin && iymax - 2 >= iymin) ? (double)CI(iymax - 2, j) : 0.0;
-        for (int i = iymax - 1; i > oymin; i--) {
-            const T o2 = (i - 2 >= oymin) ? OV(i - 2, j) : T(0);
-            const double a2 = (colin && i - 2 >= iymin) ? (double)CI(i - 2, j) : 0.0;
-            T nxt = o1;
-            double cand;
-            if (colin && i > iymin) cand = (double)cur + (sgn_of(a1) * h);
-            else cand = (double)cur + (sgn_of(nxt) * h);
-            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i - 1, j) = nxt; }
-            cur = nxt;
-            a1 = a2; o1 = o2;
-        }
+    for (int j0 = ixmin; j0 < oxmax; j0 += blockDim.x) {
+        const int j = j0 + (int)threadIdx.x;
+        const bool act = j < oxmax, colin = j < ixmax;
+        sweep_line<T, false>([&](int t) { return OV(iymax - 1 - t, j); }, [&](int t, T v) { OV(iymax - 1 - t, j) = v; },
+                             [&](int t) { return CI(iymax - 1 - t, j); }, iymax - 1 - oymin, iymax - 1 - iymin, act, colin, h);
     }
     __syncthreads();
-    // x forward (1339-1367)
+    // x forward (1339-1367), x backward (1369-1388): lanes own rows
     for (int i0 = iymin; i0 < oymax; i0 += blockDim.x) {
         const int i = i0 + (int)threadIdx.x;
-        const bool on = i < oymax && ixmin < oxmax - 1;
-        const bool rowin = i < iymax;
-        Crossings X = find_crossings([&](int j) { return (double)CI(i, j); }, ixmin, min(oxmax - 1, ixmax - 1),
-                                     on && rowin, h);
-        if (on) {
-            T cur = OV(i, ixmin);
-            T o1 = OV(i, ixmin + 1);
-            double a = (rowin && ixmin < ixmax) ? (double)CI(i, ixmin) : 0.0;
-            double a1 = (rowin && ixmin + 1 < ixmax) ? (double)CI(i, ixmin + 1) : 0.0;
-            for (int j = ixmin; j < oxmax - 1; j++) {
-                const T o2 = (j + 2 < oxmax) ? OV(i, j + 2) : T(0);
-                const double a2 = (rowin && j + 2 < ixmax) ? (double)CI(i, j + 2) : 0.0;
-                T nxt = o1;
-                double cand;
-                if (j < ixmax - 1 && rowin) {
-                    if (a * a1 <= 0.0) {
-                        double d0, sg;
-                        if (X.i0 == j) { d0 = X.d0; sg = X.s0; X.pop(); }
-                        else { const double slope = (a1 - a) / h; d0 = -a / slope; sg = (double)sgn_of(slope); }
-                        if (fabs((double)cur) > fabs(d0)) { cur = (T)(-d0 * sg); OV(i, j) = cur; }
-                    }
-                    cand = (double)cur + (sgn_of(a1) * h);
-                } else {
-                    cand = (double)cur + (sgn_of(nxt) * h);
-                }
-                if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j + 1) = nxt; }
-                cur = nxt;
-                a = a1; a1 = a2; o1 = o2;
-            }
-        }
+        const bool act = i < oymax, rowin = i < iymax;
+        sweep_line<T, true>([&](int t) { return OV(i, ixmin + t); }, [&](int t, T v) { OV(i, ixmin + t) = v; },
+                            [&](int t) { return CI(i, ixmin + t); }, oxmax - 1 - ixmin, ixmax - 1 - ixmin, act, rowin, h);
     }
     __syncthreads();
-    // x backward (1369-1388)
-    for (int i = iymin + threadIdx.x; i < oymax; i += blockDim.x) {
-        if (ixmax - 1 <= oxmin) break;
-        const bool rowin = i < iymax;
-        T cur = OV(i, ixmax - 1);
-        T o1 = OV(i, ixmax - 2);
-        double a1 = (rowin && ixmax - 2 >= ixmin) ? (double)CI(i, ixmax - 2) : 0.0;
-        for (int j = ixmax - 1; j > oxmin; j--) {
-            const T o2 = (j - 2 >= oxmin) ? OV(i, j - 2) : T(0);
-            const double a2 = (rowin && j - 2 >= ixmin) ? (double)CI(i, j - 2) : 0.0;
-            T nxt = o1;
-            double cand;
-            if (j > ixmin && rowin) cand = (double)cur + (sgn_of(a1) * h);
-            else cand = (double)cur + sgn_of(nxt) * h;
-            if (fabs(cand) < fabs((double)nxt)) { nxt = (T)cand; OV(i, j - 1) = nxt; }
-            cur = nxt;
-            a1 = a2; o1 = o2;
-        }
+    for (int i0 = iymin; i0 < oymax; i0 += blockDim.x) {
+        const int i = i0 + (int)threadIdx.x;
+        const bool act = i < oymax, rowin = i < iymax;
+        sweep_line<T, false>([&](int t) { return OV(i, ixmax - 1 - t); }, [&](int t, T v) { OV(i, ixmax - 1 - t) = v; },
+                             [&](int t) { return CI(i, ixmax - 1 - t); }, ixmax - 1 - oxmin, ixmax - 1 - ixmin, act, rowin, h);
     }
     __syncthreads();
 #undef OV
@@ -552,9 +491,9 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
             unsigned hits = 0;
             // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
-            // chunk, so the gathers of one neighbour are independent loads issued together;
-            // per pixel the neighbours are still met in list order (strict >, first wins).
-            constexpr int kPx = 8;
+            // chunk, so the gathers of a round of neighbours are independent loads issued
+            // together; per pixel the neighbours are still met in list order (strict >, first wins).
+            constexpr int kPx = 4;
             for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
                 int yy[kPx], xx[kPx], id[kPx];
                 T mx[kPx];
@@ -566,23 +505,32 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                     mx[t] = T(-1.0);
                     id[t] = -1;
                 }
-#pragma unroll 2
-                for (int k = 0; k < cnt; ++k) {
-                    const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
-                    const T* nsrc = A.tilde + s_off[k];
-                    T v[kPx];
-                    bool in[kPx];
+                // four list entries per round: their gathers are issued together, then compared
+                // in list order
+                constexpr int kK = 4;
+                for (int k0 = 0; k0 < cnt; k0 += kK) {
+                    T v[kK][kPx];
+                    bool in[kK][kPx];
 #pragma unroll
-                    for (int t = 0; t < kPx; ++t) {
-                        const int ly = yy[t] - dy, lx = xx[t] - dx;
-                        in[t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
-                        v[t] = in[t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                    for (int u = 0; u < kK; ++u) {
+                        const int k = k0 + u;
+                        const bool lk = k < cnt;
+                        const int dy = lk ? s_y0[k] - b.y0 : 0, dx = lk ? s_x0[k] - b.x0 : 0, nk = lk ? s_n[k] : 0;
+                        const T* nsrc = A.tilde + (lk ? s_off[k] : 0);
+#pragma unroll
+                        for (int t = 0; t < kPx; ++t) {
+                            const int ly = yy[t] - dy, lx = xx[t] - dx;
+                            in[u][t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
+                            v[u][t] = in[u][t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                        }
                     }
 #pragma unroll
-                    for (int t = 0; t < kPx; ++t) {
-                        hits += in[t] ? 1u : 0u;
-                        if (in[t] && v[t] > mx[t]) { mx[t] = v[t]; id[t] = s_id[k]; }
-                    }
+                    for (int u = 0; u < kK; ++u)
+#pragma unroll
+                        for (int t = 0; t < kPx; ++t) {
+                            hits += in[u][t] ? 1u : 0u;
+                            if (in[u][t] && v[u][t] > mx[t]) { mx[t] = v[u][t]; id[t] = s_id[k0 + u]; }
+                        }
                 }
 #pragma unroll
                 for (int t = 0; t < kPx; ++t) {
