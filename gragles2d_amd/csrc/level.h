@@ -176,29 +176,31 @@ __device__ __forceinline__ int block_flag_scan(bool f, int* s_wt, int* total) {
 }
 
 // Marching-squares case table of a box, one byte per square: the square (r, c), r, c in
-// [0, n], has its top-left corner at pixel (y0 - 1 + r, x0 - 1 + c).  Bits 0-1: the direction
-// of an ordinary case, SQ_GEN: the case generates a point, SQ_S6 / SQ_S9: the two saddles
-// (resolved by the previous direction), SQ_NONE: no boundary (the walk keeps its direction).
-// The cases and directions are those of marchingSquares.cpp:44-176.
-enum { SQ_GEN = 4, SQ_S6 = 8, SQ_S9 = 16, SQ_NONE = 32 };
+// [0, n], has its top-left corner at pixel (y0 - 1 + r, x0 - 1 + c).  Codes 0-3: an ordinary
+// case, which generates a point and leaves in that direction (D_UP..D_RIGHT); SQ_S6 / SQ_S9:
+// the two saddles (resolved by the previous direction, no point); SQ_NONE: no boundary (the
+// walk keeps its direction).  The cases and directions are those of marchingSquares.cpp:44-176.
+enum { SQ_S6 = 4, SQ_S9 = 5, SQ_NONE = 6 };
 constexpr int kTabMax = 128;  // largest box side walked through the table
+constexpr int kRegMax = 63;
+constexpr int kOMargin = 3;   // the LDS output tile holds new boxes up to nb + kOMargin   // largest box side whose table wave 0 keeps in registers for the walk
 
 __device__ __forceinline__ unsigned char square_code(int mask) {
     switch (mask) {
-        case MS_TL: return D_UP | SQ_GEN;
-        case MS_TR: return D_RIGHT | SQ_GEN;
-        case MS_TL | MS_TR: return D_RIGHT | SQ_GEN;
-        case MS_BL: return D_LEFT | SQ_GEN;
-        case MS_BL | MS_TL: return D_UP | SQ_GEN;
+        case MS_TL: return D_UP;
+        case MS_TR: return D_RIGHT;
+        case MS_TL | MS_TR: return D_RIGHT;
+        case MS_BL: return D_LEFT;
+        case MS_BL | MS_TL: return D_UP;
         case MS_BL | MS_TR: return SQ_S6;
-        case MS_BL | MS_TR | MS_TL: return D_RIGHT | SQ_GEN;
-        case MS_BR: return D_DOWN | SQ_GEN;
+        case MS_BL | MS_TR | MS_TL: return D_RIGHT;
+        case MS_BR: return D_DOWN;
         case MS_BR | MS_TL: return SQ_S9;
-        case MS_BR | MS_TR: return D_DOWN | SQ_GEN;
-        case MS_BR | MS_TR | MS_TL: return D_DOWN | SQ_GEN;
-        case MS_BR | MS_BL: return D_LEFT | SQ_GEN;
-        case MS_BR | MS_BL | MS_TL: return D_UP | SQ_GEN;
-        case MS_BR | MS_BL | MS_TR: return D_LEFT | SQ_GEN;
+        case MS_BR | MS_TR: return D_DOWN;
+        case MS_BR | MS_TR | MS_TL: return D_DOWN;
+        case MS_BR | MS_BL: return D_LEFT;
+        case MS_BR | MS_BL | MS_TL: return D_UP;
+        case MS_BR | MS_BL | MS_TR: return D_LEFT;
         default: return SQ_NONE;
     }
 }
@@ -219,26 +221,25 @@ __device__ __forceinline__ void build_squares(const T* C, int n, int cp, unsigne
 }
 
 // One wave walks the squares from the start pixel (sr, sc) (box-local; its square is (sr, sc)),
-// wave-uniformly: one LDS byte per move.  Moves are recorded as walk_squares records them.
-// Returns 0, kErrContourCap or kErrTrace.
-__device__ __forceinline__ int table_walk(const unsigned char* tab, int n, int sr, int sc, int ccap, uint32_t* moves,
-                                          int* nmoves) {
+// wave-uniformly; code(r, c) returns the case code of a square inside the table.  Moves are
+// recorded as walk_squares records them.  Returns 0, kErrContourCap or kErrTrace.
+template <class Code>
+__device__ __forceinline__ int table_walk(Code code, int n, int sr, int sc, int ccap, uint32_t* moves, int* nmoves) {
     const int lane = threadIdx.x & 63;
     const int w = n + 1;
     const long long itcap = 8LL * (n + 2) * (n + 2) + 16;
     long long it = 0;
     int r = sr, c = sc, next = D_NONE, prev = D_NONE, nm = 0, st = 0;
     do {
-        int e = SQ_NONE;
-        if ((unsigned)r < (unsigned)w && (unsigned)c < (unsigned)w) e = __builtin_amdgcn_readfirstlane((int)tab[r * w + c]);
+        const int e = ((unsigned)r < (unsigned)w && (unsigned)c < (unsigned)w) ? code(r, c) : SQ_NONE;
         bool gen = false;
-        if (e & SQ_NONE) {
-        } else if (e & SQ_S6) {
+        if (e == SQ_NONE) {
+        } else if (e == SQ_S6) {
             next = (prev == D_UP) ? D_RIGHT : D_LEFT;
-        } else if (e & SQ_S9) {
+        } else if (e == SQ_S9) {
             next = (prev == D_RIGHT) ? D_DOWN : D_UP;
         } else {
-            next = e & 3;
+            next = e;
             gen = true;
         }
         if (gen) {
@@ -255,6 +256,44 @@ __device__ __forceinline__ int table_walk(const unsigned char* tab, int n, int s
     return st;
 }
 
+// Square codes of a box up to kRegMax in wave 0's VGPRs: lane r holds the 64 squares of row r as
+// 4-bit codes in eight words, so a walk step is one uniform readlane, not an LDS round trip.
+struct SquareRows {
+    unsigned w0, w1, w2, w3, w4, w5, w6, w7;
+};
+
+__device__ __forceinline__ SquareRows load_square_rows(const unsigned char* tab, int n) {
+    const int r = threadIdx.x & 63, w = n + 1;
+    unsigned v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        unsigned x = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = 8 * k + q;
+            const unsigned code = (r < w && c < w) ? (unsigned)tab[r * w + c] : (unsigned)SQ_NONE;
+            x |= code << (4 * q);
+        }
+        v[k] = x;
+    }
+    return SquareRows{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+}
+
+__device__ __forceinline__ int square_from_rows(const SquareRows& R, int r, int c) {
+    unsigned x;
+    switch (c >> 3) {  // uniform: every choice is a readlane of a named register
+        case 0: x = (unsigned)__builtin_amdgcn_readlane((int)R.w0, r); break;
+        case 1: x = (unsigned)__builtin_amdgcn_readlane((int)R.w1, r); break;
+        case 2: x = (unsigned)__builtin_amdgcn_readlane((int)R.w2, r); break;
+        case 3: x = (unsigned)__builtin_amdgcn_readlane((int)R.w3, r); break;
+        case 4: x = (unsigned)__builtin_amdgcn_readlane((int)R.w4, r); break;
+        case 5: x = (unsigned)__builtin_amdgcn_readlane((int)R.w5, r); break;
+        case 6: x = (unsigned)__builtin_amdgcn_readlane((int)R.w6, r); break;
+        default: x = (unsigned)__builtin_amdgcn_readlane((int)R.w7, r); break;
+    }
+    return (int)((x >> (4 * (c & 7))) & 15u);
+}
+
 // LDS layout of the fused kernel for boxes up to side nb.  Full layout: the C tile, then the O
 // tile, which until redistancing hosts the per-point work arrays (36 bytes per contour point:
 // the compacted contour; the interpolated moves, later the segment lengths and shoelace
@@ -268,7 +307,7 @@ template <class T>
 __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     LevelLayout L;
     const long long ccap = contour_cap(nb);
-    const long long no = nb + 8;
+    const long long no = nb + kOMargin;
     const long long tab = nb <= kTabMax ? (long long)(nb + 1) * (nb + 1) : 0;
     const long long work = (ccap * 36 + tab + 15) & ~15LL;
     L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
@@ -443,8 +482,12 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
     __shared__ int s_bb[4];
     __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len
     __shared__ double s_dbl[8];   // 0 volume sum, 1 minx, 2 maxx, 3 miny, 4 maxy
-    __shared__ int s_id[kCmpCap], s_x0[kCmpCap], s_y0[kCmpCap], s_n[kCmpCap];
-    __shared__ long long s_off[kCmpCap];
+    // the comparison list's boxes, in the output tile's space (free until the contour)
+    long long* s_off = (long long*)obase;
+    int* s_id = (int*)(s_off + kCmpCap);
+    int* s_x0 = s_id + kCmpCap;
+    int* s_y0 = s_x0 + kCmpCap;
+    int* s_n = s_y0 + kCmpCap;
     __shared__ long long s_coff, s_onext;
     __shared__ Box s_nbx;
     __shared__ unsigned s_hits;
@@ -500,7 +543,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
 #pragma unroll
                 for (int t = 0; t < kPx; ++t) {
                     const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
-                    yy[t] = p < nn ? (int)(p / n) : n;  // row n lies outside every overlap
+                    yy[t] = p < nn ? (int)(p / n) : -(1 << 20);  // padding: a row no box covers
                     xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
                     mx[t] = T(-1.0);
                     id[t] = -1;
@@ -609,8 +652,16 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                 if (tabled) {
                     if (threadIdx.x < 64) {
                         const int start = __builtin_amdgcn_readfirstlane(s_int[0]);
-                        int nm = 0;
-                        const int st = table_walk(s_tab, n, start / n, start % n, ccap, moves, &nm);
+                        int nm = 0, st;
+                        if (n <= kRegMax) {
+                            const SquareRows R = load_square_rows(s_tab, n);
+                            st = table_walk([&](int r, int c) { return square_from_rows(R, r, c); }, n, start / n,
+                                            start % n, ccap, moves, &nm);
+                        } else {
+                            const int w = n + 1;
+                            st = table_walk([&](int r, int c) { return __builtin_amdgcn_readfirstlane((int)s_tab[r * w + c]); },
+                                            n, start / n, start % n, ccap, moves, &nm);
+                        }
                         if (threadIdx.x == 0) {
                             s_int[1] = nm;
                             if (st) s_int[3] = st;
@@ -909,7 +960,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const Box bo = s_nbx;
             T* gout = A.out_field + s_onext;
             if constexpr (FULL) {
-                if (bo.n <= nb + 8) redistance<T>(Ol, bo.n | 1, C, cp, b, bo, G, gout);
+                if (bo.n <= nb + kOMargin) redistance<T>(Ol, bo.n | 1, C, cp, b, bo, G, gout);
                 else redistance<T>(gout, bo.n, C, cp, b, bo, G, gout);
             } else {
                 redistance<T>(gout, bo.n, C, cp, b, bo, G, gout);
