@@ -182,8 +182,8 @@ __device__ __forceinline__ int block_flag_scan(bool f, int* s_wt, int* total) {
 // walk keeps its direction).  The cases and directions are those of marchingSquares.cpp:44-176.
 enum { SQ_S6 = 4, SQ_S9 = 5, SQ_NONE = 6 };
 constexpr int kTabMax = 128;  // largest box side walked through the table
-constexpr int kRegMax = 63;
-constexpr int kOMargin = 3;   // the LDS output tile holds new boxes up to nb + kOMargin   // largest box side whose table wave 0 keeps in registers for the walk
+constexpr int kRegMax = 63;   // largest box side walked through the chain table
+constexpr int kOMargin = 3;   // the LDS output tile holds new boxes up to nb + kOMargin
 
 __device__ __forceinline__ unsigned char square_code(int mask) {
     switch (mask) {
@@ -256,42 +256,93 @@ __device__ __forceinline__ int table_walk(Code code, int n, int sr, int sc, int 
     return st;
 }
 
-// Square codes of a box up to kRegMax in wave 0's VGPRs: lane r holds the 64 squares of row r as
-// 4-bit codes in eight words, so a walk step is one uniform readlane, not an LDS round trip.
-struct SquareRows {
-    unsigned w0, w1, w2, w3, w4, w5, w6, w7;
-};
+// Chains of ordinary squares, for boxes up to kRegMax: entry q of the chain table holds the
+// moves the walk makes from square q until it meets a saddle, a square without boundary, the
+// start square or the edge of the table, at most kChain of them: bits 0-3 the count, then two
+// bits per move direction, and in bits 28-30 the square's own code.  Built by all lanes, each
+// following the chains of kGrp squares together, so the dependent table reads of a chain cost
+// one LDS round trip per step for all of them; the walk then advances kChain moves per read.
+constexpr int kChain = 8;
 
-__device__ __forceinline__ SquareRows load_square_rows(const unsigned char* tab, int n) {
-    const int r = threadIdx.x & 63, w = n + 1;
-    unsigned v[8];
+__device__ __forceinline__ void build_chains(const unsigned char* tab, int n, int start_sq, uint32_t* chain) {
+    constexpr int kGrp = 8;
+    const int w = n + 1, W = w * w;
+    for (int q0 = 0; q0 < W; q0 += kGrp * (int)blockDim.x) {
+        int r[kGrp], c[kGrp], m[kGrp];
+        uint32_t bits[kGrp];
+        bool live[kGrp];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        unsigned x = 0u;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int c = 8 * k + q;
-            const unsigned code = (r < w && c < w) ? (unsigned)tab[r * w + c] : (unsigned)SQ_NONE;
-            x |= code << (4 * q);
+        for (int g = 0; g < kGrp; ++g) {
+            const int q = q0 + g * (int)blockDim.x + (int)threadIdx.x;
+            live[g] = q < W;
+            r[g] = live[g] ? q / w : 0;
+            c[g] = live[g] ? q - r[g] * w : 0;
+            m[g] = 0;
+            bits[g] = 0u;
         }
-        v[k] = x;
+        for (int k = 0; k < kChain; ++k) {
+            unsigned code[kGrp];
+#pragma unroll
+            for (int g = 0; g < kGrp; ++g) code[g] = live[g] ? (unsigned)tab[r[g] * w + c[g]] : (unsigned)SQ_NONE;
+#pragma unroll
+            for (int g = 0; g < kGrp; ++g) {
+                if (k == 0) bits[g] |= code[g] << 28;
+                if (!live[g]) continue;
+                if (code[g] >= (unsigned)SQ_S6) { live[g] = false; continue; }
+                bits[g] |= code[g] << (4 + 2 * k);
+                m[g]++;
+                r[g] += (code[g] == D_DOWN) - (code[g] == D_UP);
+                c[g] += (code[g] == D_RIGHT) - (code[g] == D_LEFT);
+                if ((unsigned)r[g] >= (unsigned)w || (unsigned)c[g] >= (unsigned)w || r[g] * w + c[g] == start_sq)
+                    live[g] = false;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kGrp; ++g) {
+            const int q = q0 + g * (int)blockDim.x + (int)threadIdx.x;
+            if (q < W) chain[q] = bits[g] | (uint32_t)m[g];
+        }
     }
-    return SquareRows{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
 }
 
-__device__ __forceinline__ int square_from_rows(const SquareRows& R, int r, int c) {
-    unsigned x;
-    switch (c >> 3) {  // uniform: every choice is a readlane of a named register
-        case 0: x = (unsigned)__builtin_amdgcn_readlane((int)R.w0, r); break;
-        case 1: x = (unsigned)__builtin_amdgcn_readlane((int)R.w1, r); break;
-        case 2: x = (unsigned)__builtin_amdgcn_readlane((int)R.w2, r); break;
-        case 3: x = (unsigned)__builtin_amdgcn_readlane((int)R.w3, r); break;
-        case 4: x = (unsigned)__builtin_amdgcn_readlane((int)R.w4, r); break;
-        case 5: x = (unsigned)__builtin_amdgcn_readlane((int)R.w5, r); break;
-        case 6: x = (unsigned)__builtin_amdgcn_readlane((int)R.w6, r); break;
-        default: x = (unsigned)__builtin_amdgcn_readlane((int)R.w7, r); break;
-    }
-    return (int)((x >> (4 * (c & 7))) & 15u);
+// The walk of table_walk over the chain table (wave-uniform; same moves, same errors).
+__device__ __forceinline__ int chain_walk(const uint32_t* chain, int n, int sr, int sc, int ccap, uint32_t* moves,
+                                          int* nmoves) {
+    const int lane = threadIdx.x & 63;
+    const int w = n + 1;
+    const long long itcap = 8LL * (n + 2) * (n + 2) + 16;
+    long long it = 0;
+    int r = sr, c = sc, next = D_NONE, prev = D_NONE, nm = 0, st = 0;
+    do {
+        const uint32_t e = ((unsigned)r < (unsigned)w && (unsigned)c < (unsigned)w)
+                               ? (uint32_t)__builtin_amdgcn_readfirstlane((int)chain[r * w + c])
+                               : ((uint32_t)SQ_NONE << 28);
+        const int m = (int)(e & 15u);
+        if (m == 0) {
+            const int code = (int)((e >> 28) & 7u);
+            if (code == SQ_S6) next = (prev == D_UP) ? D_RIGHT : D_LEFT;
+            else if (code == SQ_S9) next = (prev == D_RIGHT) ? D_DOWN : D_UP;
+            r += (next == D_DOWN) - (next == D_UP);
+            c += (next == D_RIGHT) - (next == D_LEFT);
+            prev = next;
+            ++it;
+        } else {
+            for (int k = 0; k < m; ++k) {
+                const int d = (int)((e >> (4 + 2 * k)) & 3u);
+                if (nm >= ccap) { st = kErrContourCap; break; }
+                if (lane == 0) moves[nm] = (uint32_t)d | ((uint32_t)c << 2) | ((uint32_t)r << 16);
+                nm++;
+                r += (d == D_DOWN) - (d == D_UP);
+                c += (d == D_RIGHT) - (d == D_LEFT);
+                next = prev = d;
+            }
+            if (st) break;
+            it += m;
+        }
+        if (it > itcap) { st = kErrTrace; break; }
+    } while (c != sc || r != sr);
+    *nmoves = nm;
+    return st;
 }
 
 // LDS layout of the fused kernel for boxes up to side nb.  Full layout: the C tile, then the O
@@ -648,15 +699,17 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const bool tabled = nb <= kTabMax;  // the layout holds a table for buckets up to kTabMax
             if (tabled) build_squares(C, n, cp, s_tab);
             __syncthreads();
+            if (tabled && n <= kRegMax && s_int[0] != 0x7fffffff) {  // chain table in the (free) contour space
+                build_chains(s_tab, n, (s_int[0] / n) * (n + 1) + s_int[0] % n, (uint32_t*)s_pt);
+                __syncthreads();
+            }
             if (s_int[0] != 0x7fffffff) {
                 if (tabled) {
                     if (threadIdx.x < 64) {
                         const int start = __builtin_amdgcn_readfirstlane(s_int[0]);
                         int nm = 0, st;
                         if (n <= kRegMax) {
-                            const SquareRows R = load_square_rows(s_tab, n);
-                            st = table_walk([&](int r, int c) { return square_from_rows(R, r, c); }, n, start / n,
-                                            start % n, ccap, moves, &nm);
+                            st = chain_walk((const uint32_t*)s_pt, n, start / n, start % n, ccap, moves, &nm);
                         } else {
                             const int w = n + 1;
                             st = table_walk([&](int r, int c) { return __builtin_amdgcn_readfirstlane((int)s_tab[r * w + c]); },
