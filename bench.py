@@ -71,6 +71,7 @@ def main():
     else:
         kern, byts, ms = "k_conv_dense (convolution)", conv_b, phases["convolution_ms"]
     achieved = byts / (ms * 1e-3)
+    traffic = measured_traffic(w.name, a.precision, kern)
     out = {
         "metric": "grid-point updates/s (batched timestep)",
         "value": value,
@@ -90,7 +91,7 @@ def main():
         "phases_ms_per_step": phases,
         "algorithmic_bytes_per_step": {"convolution": conv_b, "level": level_b},
         "roofline": {"kernel": kern, "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic},
         "step_roofline_frac": (conv_b + level_b) / (sec / a.steps) / HBM_PEAK,
     }
     hdl.close()
@@ -111,6 +112,20 @@ def main():
     if a.cpu_grains > 0:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_grains, a.cpu_steps)
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per step of the dominant phase from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by what the counters read for a copy of known size);
+    None when no summary exists for this workload and precision."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload.replace('-', '')}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    name = "ggx::k_level<double, true>" if kernel.startswith("k_level") else "ggx::k_conv_dense<double>"
+    e = t.get("per_step", {}).get(name)
+    return None if e is None else e["hbm_bytes"]
 
 
 def cpu_baseline(w, ngrains, steps):
