@@ -364,6 +364,10 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
     L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
     if (work > L.otile) L.otile = work;
+    // the full layout's comparison keeps the list boxes (24 bytes per entry) and a list position
+    // per pixel in the output tile's space
+    const long long cmpw = 24LL * kCmpCap + 4LL * nb * nb;
+    if (full && cmpw > L.otile) L.otile = (cmpw + 15) & ~15LL;
     L.total = L.ctile + L.otile;
     return L;
 }
@@ -584,78 +588,127 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
             const double band = 0.7 * G.delta;
             int ymin_l = 0x7fffffff, ymax_l = -0x7fffffff, xmin_l = 0x7fffffff, xmax_l = -0x7fffffff;
             unsigned hits = 0;
-            // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
-            // chunk, so the gathers of a round of neighbours are independent loads issued
-            // together; per pixel the neighbours are still met in list order (strict >, first wins).
-            constexpr int kPx = 4;
-            for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
-                int yy[kPx], xx[kPx], id[kPx];
-                T mx[kPx];
-#pragma unroll
-                for (int t = 0; t < kPx; ++t) {
-                    const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
-                    yy[t] = p < nn ? (int)(p / n) : -(1 << 20);  // padding: a row no box covers
-                    xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
-                    mx[t] = T(-1.0);
-                    id[t] = -1;
+            // per pixel, after the gathers: the wall as grain 0 (boundaryCondition), IDLocal,
+            // the set comparison and the bounding box of C >= 0
+            auto finish = [&](long long p, int yy, int xx, T m, int idt) {
+                const int i = b.y0 + yy, j = b.x0 + xx;
+                if (bnd) {
+                    int gb = G.gb, m2 = G.m;
+                    int dXMin = -(j - gb), dYMin = -(i - gb);
+                    int dXMax = (j - (m2 - gb)), dYMax = (i - (m2 - gb));
+                    int dX = (abs(dXMin) < abs(dXMax)) ? dXMin : dXMax;
+                    int dY = (abs(dYMin) < abs(dYMax)) ? dYMin : dYMax;
+                    double dist;
+                    if (dX > 0 && dY > 0) dist = sqrt((double)dX * dX + dY * dY);
+                    else if (dX < 0 && dY > 0) dist = dY;
+                    else if (dX > 0 && dY < 0) dist = dX;
+                    else if (dX < 0 && dY < 0) dist = max(dX, dY);
+                    else if (dX == 0) dist = (dY > 0) ? (double)dY : 0.0;
+                    else dist = (dX > 0) ? (double)dX : 0.0;
+                    if ((T)(dist * G.h) > m) { m = (T)(dist * G.h); idt = 0; }
                 }
-                // four list entries per round: their gathers are issued together, then compared
-                // in list order
-                constexpr int kK = 4;
-                for (int k0 = 0; k0 < cnt; k0 += kK) {
-                    T v[kK][kPx];
-                    bool in[kK][kPx];
+                uint32_t idv = (idt >= 0) ? (uint32_t)idt : ((p < np) ? A.idl_old[op + p] : 0u);
+                A.idl_new[o + p] = idv;
+                T tv = A.tilde[o + p];
+                T c = (fabs((double)tv) < band) ? (T)(0.5 * ((double)tv - (double)m)) : tv;
+                C[yy * cp + xx] = c;
+                if (c >= T(0)) {
+                    ymin_l = min(ymin_l, i); ymax_l = max(ymax_l, i);
+                    xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
+                }
+            };
+            if constexpr (FULL) {
+                // Owner computes over each entry's overlap rectangle: wave w owns a block of rows,
+                // lane l column l (+64...), so every pixel is updated by one lane, in list order
+                // (strict >, first wins), and the work is the overlap area, not list x box.
+                // The running maximum lives in the C tile, the winning list position in the
+                // output tile's space after the list boxes.
+                T* M = C;
+                int* ID = s_n + kCmpCap;
+                for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
+                    const int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
+                    M[yy * cp + xx] = T(-1.0);
+                    ID[p] = -1;
+                }
+                __syncthreads();
+                const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), nw = (int)(blockDim.x >> 6);
+                const int R = (n + nw - 1) / nw, yb0 = wave * R, yb1 = min(n, yb0 + R);
+                for (int xg = 0; xg < n; xg += 64) {
+                    const int x = xg + lane;
+                    for (int k = 0; k < cnt; ++k) {
+                        const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
+                        const int r0 = max(max(dy, 0), yb0), r1 = min(min(dy + nk, n), yb1);
+                        if (r0 >= r1) continue;
+                        const bool colin = x < n && (unsigned)(x - dx) < (unsigned)nk;
+                        const T* nsrc = A.tilde + s_off[k] + (x - dx) - (long long)dy * nk;
+                        for (int r = r0; r < r1; r += 4) {
+                            T v[4];
+                            bool ok[4];
 #pragma unroll
-                    for (int u = 0; u < kK; ++u) {
-                        const int k = k0 + u;
-                        const bool lk = k < cnt;
-                        const int dy = lk ? s_y0[k] - b.y0 : 0, dx = lk ? s_x0[k] - b.x0 : 0, nk = lk ? s_n[k] : 0;
-                        const T* nsrc = A.tilde + (lk ? s_off[k] : 0);
+                            for (int u = 0; u < 4; ++u) {
+                                ok[u] = colin && r + u < r1;
+                                v[u] = ok[u] ? nsrc[(long long)(r + u) * nk] : T(0);
+                            }
 #pragma unroll
-                        for (int t = 0; t < kPx; ++t) {
-                            const int ly = yy[t] - dy, lx = xx[t] - dx;
-                            in[u][t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
-                            v[u][t] = in[u][t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                            for (int u = 0; u < 4; ++u) {
+                                if (!ok[u]) continue;
+                                hits++;
+                                const int pm = (r + u) * cp + x;
+                                if (v[u] > M[pm]) { M[pm] = v[u]; ID[(r + u) * n + x] = k; }
+                            }
                         }
                     }
-#pragma unroll
-                    for (int u = 0; u < kK; ++u)
-#pragma unroll
-                        for (int t = 0; t < kPx; ++t) {
-                            hits += in[u][t] ? 1u : 0u;
-                            if (in[u][t] && v[u][t] > mx[t]) { mx[t] = v[u][t]; id[t] = s_id[k0 + u]; }
-                        }
                 }
+                __syncthreads();
+                for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
+                    const int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
+                    const int kk = ID[p];
+                    finish(p, yy, xx, M[yy * cp + xx], kk >= 0 ? s_id[kk] : -1);
+                }
+            } else {
+                // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
+                // chunk, in rounds of kK entries whose gathers are issued together; per pixel the
+                // entries are met in list order (strict >, first wins).
+                constexpr int kPx = 4, kK = 4;
+                for (long long c0 = 0; c0 < nn; c0 += (long long)kPx * blockDim.x) {
+                    int yy[kPx], xx[kPx], id[kPx];
+                    T mx[kPx];
 #pragma unroll
-                for (int t = 0; t < kPx; ++t) {
-                    const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
-                    if (p >= nn) continue;
-                    const int i = b.y0 + yy[t], j = b.x0 + xx[t];
-                    T m = mx[t];
-                    int idt = id[t];
-                    if (bnd) {
-                        int gb = G.gb, m2 = G.m;
-                        int dXMin = -(j - gb), dYMin = -(i - gb);
-                        int dXMax = (j - (m2 - gb)), dYMax = (i - (m2 - gb));
-                        int dX = (abs(dXMin) < abs(dXMax)) ? dXMin : dXMax;
-                        int dY = (abs(dYMin) < abs(dYMax)) ? dYMin : dYMax;
-                        double dist;
-                        if (dX > 0 && dY > 0) dist = sqrt((double)dX * dX + dY * dY);
-                        else if (dX < 0 && dY > 0) dist = dY;
-                        else if (dX > 0 && dY < 0) dist = dX;
-                        else if (dX < 0 && dY < 0) dist = max(dX, dY);
-                        else if (dX == 0) dist = (dY > 0) ? (double)dY : 0.0;
-                        else dist = (dX > 0) ? (double)dX : 0.0;
-                        if ((T)(dist * G.h) > m) { m = (T)(dist * G.h); idt = 0; }
+                    for (int t = 0; t < kPx; ++t) {
+                        const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
+                        yy[t] = p < nn ? (int)(p / n) : -(1 << 20);  // padding: a row no box covers
+                        xx[t] = p < nn ? (int)(p - (long long)yy[t] * n) : 0;
+                        mx[t] = T(-1.0);
+                        id[t] = -1;
                     }
-                    uint32_t idv = (idt >= 0) ? (uint32_t)idt : ((p < np) ? A.idl_old[op + p] : 0u);
-                    A.idl_new[o + p] = idv;
-                    T tv = A.tilde[o + p];
-                    T c = (fabs((double)tv) < band) ? (T)(0.5 * ((double)tv - (double)m)) : tv;
-                    C[yy[t] * cp + xx[t]] = c;
-                    if (c >= T(0)) {
-                        ymin_l = min(ymin_l, i); ymax_l = max(ymax_l, i);
-                        xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
+                    for (int k0 = 0; k0 < cnt; k0 += kK) {
+                        T v[kK][kPx];
+                        bool in[kK][kPx];
+#pragma unroll
+                        for (int u = 0; u < kK; ++u) {
+                            const int k = k0 + u;
+                            const bool lk = k < cnt;
+                            const int dy = lk ? s_y0[k] - b.y0 : 0, dx = lk ? s_x0[k] - b.x0 : 0, nk = lk ? s_n[k] : 0;
+                            const T* nsrc = A.tilde + (lk ? s_off[k] : 0);
+#pragma unroll
+                            for (int t = 0; t < kPx; ++t) {
+                                const int ly = yy[t] - dy, lx = xx[t] - dx;
+                                in[u][t] = (unsigned)ly < (unsigned)nk && (unsigned)lx < (unsigned)nk;
+                                v[u][t] = in[u][t] ? nsrc[(long long)ly * nk + lx] : T(0);
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kK; ++u)
+#pragma unroll
+                            for (int t = 0; t < kPx; ++t) {
+                                hits += in[u][t] ? 1u : 0u;
+                                if (in[u][t] && v[u][t] > mx[t]) { mx[t] = v[u][t]; id[t] = s_id[k0 + u]; }
+                            }
+                    }
+#pragma unroll
+                    for (int t = 0; t < kPx; ++t) {
+                        const long long p = c0 + threadIdx.x + (long long)t * blockDim.x;
+                        if (p < nn) finish(p, yy[t], xx[t], mx[t], id[t]);
                     }
                 }
             }
