@@ -308,7 +308,7 @@ int launch_step(ggx_ctx* ctx) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(32), 0, st, S);
     hipLaunchKernelGGL(k_prep, dim3((NS + 255) / 256), dim3(256), 0, st, NS, ctx->flags, ctx->box, bucket_table(),
                        ctx->blists, ctx->alist, ctx->errs, S);
-    hipLaunchKernelGGL(k_lists, dim3(cus * 4), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
+    hipLaunchKernelGGL(k_lists, dim3(cus * 8), dim3(256), 0, st, ctx->alist, &S->all_count, &S->all_cursor, ctx->box,
                        ctx->flags, ctx->sec_cnt, ctx->sec, ctx->cmp_cnt, ctx->cmp, ctx->errs);
     HIPCHK(hipEventRecord(ev[1], st));
     // grid per bucket: persistent (fixed) or one workgroup per grain (counts read back)

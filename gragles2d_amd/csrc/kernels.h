@@ -143,25 +143,31 @@ __device__ __forceinline__ int next_item(int* cursor, int* s_slot) {
 // first-occurrence test is an LDS hash (grain id -> smallest candidate index), so the
 // candidates are examined in parallel and compacted in their original order.
 // ------------------------------------------------------------------------------------------
-constexpr int kHash = 4096;      // LDS hash slots (grain id -> smallest candidate index)
-constexpr int kHashFill = 3072;  // distinct keys allowed per grain (keeps probing bounded)
+constexpr int kHash = 1024;      // LDS hash slots (grain id -> smallest candidate index)
+constexpr int kHashFill = 768;   // distinct keys allowed per grain (> 2 kCmpCap: beyond it the list overflows anyway)
 constexpr int kCandChunk = 2048; // neighbours-of-neighbours staged in LDS at a time
 
 // Block-wide exclusive scan of one int per lane; returns the exclusive prefix, *total gets the sum.
+// Shuffles within each wave, one exchange of wave totals through LDS (s_tmp: one int per wave).
 __device__ __forceinline__ int block_scan(int v, int* s_tmp, int* total) {
-    const int t = threadIdx.x;
-    s_tmp[t] = v;
-    __syncthreads();
-    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
-        int add = (t >= d) ? s_tmp[t - d] : 0;
-        __syncthreads();
-        s_tmp[t] += add;
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
     }
-    int incl = s_tmp[t];
-    *total = s_tmp[blockDim.x - 1];
+    if (lane == 63) s_tmp[wave] = x;
     __syncthreads();
-    return incl - v;
+    int base = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        const int c = s_tmp[w];
+        if (w < wave) base += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
 }
 
 // Open-addressing insert (key-CAS): returns the slot holding `key`, or -1 when the table holds

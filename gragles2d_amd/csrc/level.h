@@ -364,10 +364,6 @@ __host__ __device__ inline LevelLayout level_layout(int nb, bool full) {
     L.ctile = full ? (((long long)nb * (nb | 1) * (long long)sizeof(T) + 15) & ~15LL) : 0;
     L.otile = full ? ((((no | 1) * no * (long long)sizeof(T)) + 15) & ~15LL) : 0;
     if (work > L.otile) L.otile = work;
-    // the full layout's comparison keeps the list boxes (24 bytes per entry) and a list position
-    // per pixel in the output tile's space
-    const long long cmpw = 24LL * kCmpCap + 4LL * nb * nb;
-    if (full && cmpw > L.otile) L.otile = (cmpw + 15) & ~15LL;
     L.total = L.ctile + L.otile;
     return L;
 }
@@ -617,55 +613,7 @@ __global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
                     xmin_l = min(xmin_l, j); xmax_l = max(xmax_l, j);
                 }
             };
-            if constexpr (FULL) {
-                // Owner computes over each entry's overlap rectangle: wave w owns a block of rows,
-                // lane l column l (+64...), so every pixel is updated by one lane, in list order
-                // (strict >, first wins), and the work is the overlap area, not list x box.
-                // The running maximum lives in the C tile, the winning list position in the
-                // output tile's space after the list boxes.
-                T* M = C;
-                int* ID = s_n + kCmpCap;
-                for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
-                    const int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
-                    M[yy * cp + xx] = T(-1.0);
-                    ID[p] = -1;
-                }
-                __syncthreads();
-                const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), nw = (int)(blockDim.x >> 6);
-                const int R = (n + nw - 1) / nw, yb0 = wave * R, yb1 = min(n, yb0 + R);
-                for (int xg = 0; xg < n; xg += 64) {
-                    const int x = xg + lane;
-                    for (int k = 0; k < cnt; ++k) {
-                        const int dy = s_y0[k] - b.y0, dx = s_x0[k] - b.x0, nk = s_n[k];
-                        const int r0 = max(max(dy, 0), yb0), r1 = min(min(dy + nk, n), yb1);
-                        if (r0 >= r1) continue;
-                        const bool colin = x < n && (unsigned)(x - dx) < (unsigned)nk;
-                        const T* nsrc = A.tilde + s_off[k] + (x - dx) - (long long)dy * nk;
-                        for (int r = r0; r < r1; r += 4) {
-                            T v[4];
-                            bool ok[4];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                ok[u] = colin && r + u < r1;
-                                v[u] = ok[u] ? nsrc[(long long)(r + u) * nk] : T(0);
-                            }
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                if (!ok[u]) continue;
-                                hits++;
-                                const int pm = (r + u) * cp + x;
-                                if (v[u] > M[pm]) { M[pm] = v[u]; ID[(r + u) * n + x] = k; }
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-                for (long long p = threadIdx.x; p < nn; p += blockDim.x) {
-                    const int yy = (int)(p / n), xx = (int)(p - (long long)yy * n);
-                    const int kk = ID[p];
-                    finish(p, yy, xx, M[yy * cp + xx], kk >= 0 ? s_id[kk] : -1);
-                }
-            } else {
+            {
                 // Each lane keeps kPx pixels of a chunk in registers and walks the list once per
                 // chunk, in rounds of kK entries whose gathers are issued together; per pixel the
                 // entries are met in list order (strict >, first wins).

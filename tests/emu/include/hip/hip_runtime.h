@@ -142,6 +142,10 @@ inline unsigned long long __ballot(int p) { return emu::wave_exchange(p ? 1ull :
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
 inline double __longlong_as_double(long long v) { double r; std::memcpy(&r, &v, 8); return r; }
+inline int __shfl_up(int v, unsigned d) {
+    const int lane = emu::g_cur % 64;
+    return (int)emu::wave_exchange((unsigned)v, lane >= (int)d ? lane - (int)d : lane, false);
+}
 inline double __shfl_xor(double v, int mask) {
     return __longlong_as_double((long long)emu::wave_exchange((unsigned long long)__double_as_longlong(v),
                                                              (emu::g_cur % 64) ^ mask, false));
