@@ -30,7 +30,7 @@ using namespace ggx;
 
 namespace {
 
-const int kBucketEdges[] = {24, 32, 40, 48, 56, 64, 72, 88, 112, 144, 192, 256, 512};
+const int kBucketEdges[] = {24, 32, 40, 48, 56, 64, 96, 128, 192, 256, 512};
 constexpr int kNumBuckets = sizeof(kBucketEdges) / sizeof(kBucketEdges[0]);
 static_assert(kNumBuckets <= kMaxBuckets, "bucket table");
 constexpr int kMaxKernN = 2048;  // largest box side with a precomputed circulant

@@ -101,3 +101,23 @@ def test_fp32_areas_within_1e5():
             rel.append(abs(gi.volume - oi.volume) / oi.volume)
     print("fp32 max rel area error", max(rel))
     assert max(rel) <= 1e-5
+
+
+def test_persistent_launches_match_one_workgroup_per_grain_bitwise():
+    """The fixed-grid launch mode (grains pulled from the device work lists) gives the same bits."""
+    net = G.Network.generate(120, 21, 0.35)
+    s = G.default_settings(number_of_particles=len(net), number_of_points_per_grain=20, domain_border_size=7,
+                           convolution_mode=2, is_isotropic_network=1, analysis_timestep=1)
+    a = G.GrainHandler(s, batching=N.BATCH_FUSED)
+    b = G.GrainHandler(s, batching=N.BATCH_FUSED, persistent=1)
+    a.set_simulation_parameter(net)
+    b.set_simulation_parameter(net)
+    for _ in range(5):
+        a.step(1)
+        b.step(1)
+    for g in range(1, len(net) + 1):
+        ia, ib = a.info(g), b.info(g)
+        assert (ia.present, ia.exists, ia.xmin, ia.ymin, ia.xmax) == (ib.present, ib.exists, ib.xmin, ib.ymin, ib.xmax)
+        if ia.present and ia.exists:
+            assert np.array_equal(a.phi(g), b.phi(g))
+            assert list(a.neighbours(g)[0]) == list(b.neighbours(g)[0])
