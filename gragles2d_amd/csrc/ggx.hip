@@ -213,16 +213,8 @@ int raise_lds(ggx_ctx* ctx) {
     // allow the dynamic-LDS kernels to use up to `lim`
     const void* fns[] = {(const void*)k_conv_circulant<double>, (const void*)k_conv_circulant<float>,
                          (const void*)k_level<double, true>, (const void*)k_level<float, true>,
-                         (const void*)k_conv_dense<double, 8>, (const void*)k_conv_dense<float, 8>,
+                         (const void*)k_conv_dense<double>, (const void*)k_conv_dense<float>,
                          (const void*)k_level<double, false>, (const void*)k_level<float, false>};
-    const void* more[] = {(const void*)k_conv_dense<double, 4>, (const void*)k_conv_dense<float, 4>};
-    for (const void* f : more) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            lim = std::min(lim, (int)prop.sharedMemPerBlock);
-        }
-    }
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) {
@@ -281,12 +273,8 @@ int launch_conv(ggx_ctx* ctx, hipStream_t st, const int* list, const int* count,
     const long long dense = dense_lds<T>(nb);
     if (dense + ctx->static_dense[ti] + 256 <= ctx->lds_max) {
         if (grid <= 0) grid = ctx->cu_count * per_cu(ctx, dense + ctx->static_dense[ti]);
-        if (nb <= 64)
-            hipLaunchKernelGGL((k_conv_dense<T, 4>), dim3(grid), dim3(256), (size_t)dense, st, list, count, cursor,
-                               ctx->box, ctx->off, src, dst, ctx->kern_off, (const T*)ctx->kern);
-        else
-            hipLaunchKernelGGL((k_conv_dense<T, 8>), dim3(grid), dim3(256), (size_t)dense, st, list, count, cursor,
-                               ctx->box, ctx->off, src, dst, ctx->kern_off, (const T*)ctx->kern);
+        hipLaunchKernelGGL(k_conv_dense<T>, dim3(grid), dim3(256), (size_t)dense, st, list, count, cursor, ctx->box,
+                           ctx->off, src, dst, ctx->kern_off, (const T*)ctx->kern);
         return GGX_OK;
     }
     const int conv_max = ctx->lds_max - ctx->static_conv[ti] - 256;

@@ -476,22 +476,19 @@ __device__ __forceinline__ void store8(T* p, const T* a) {
     }
 }
 
-// acc[q] += sum_u v[u] * W[q - u + 7] for OB outputs and 8 inputs, W = OB + 7 consecutive
-// (wave-uniform) table entries
-template <class T, int OB>
-__device__ __forceinline__ void block_acc(T* acc, const T* v, ctab_ptr<T> w) {
-    T W[OB + 7];
+// acc[q] += sum_u v[u] * W[q - u + 7], W = 15 consecutive (wave-uniform) table entries
+template <class T>
+__device__ __forceinline__ void block8(T* acc, const T* v, ctab_ptr<T> w) {
+    T W[15];
 #pragma unroll
-    for (int t = 0; t < OB + 7; ++t) W[t] = w[t];
+    for (int t = 0; t < 15; ++t) W[t] = w[t];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-        for (int q = 0; q < OB; ++q) acc[q] += v[u] * W[q - u + 7];
+        for (int q = 0; q < 8; ++q) acc[q] += v[u] * W[q - u + 7];
 }
 
-// OB = outputs per lane and task (8, or 4 for boxes up to 64 so that the tasks of a pass spread
-// evenly over the four waves)
-template <class T, int OB>
+template <class T>
 __global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list, const int* __restrict__ count,
                                                     int* __restrict__ cursor, const Box* __restrict__ box,
                                                     const long long* __restrict__ off, const T* __restrict__ src,
@@ -521,13 +518,13 @@ __global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list
         }
         __syncthreads();
         // row pass
-        const int xb = n8 / OB, rg = (n8 + 63) >> 6;
+        const int xb = n8 >> 3, rg = (n8 + 63) >> 6;
         for (int task = wave; task < rg * xb; task += nwaves) {
             const int r64 = task / xb, bx = task - r64 * xb;
-            const int y = r64 * 64 + lane, x0 = bx * OB;
-            T acc[OB];
+            const int y = r64 * 64 + lane, x0 = bx * 8;
+            T acc[8];
 #pragma unroll
-            for (int q = 0; q < OB; ++q) acc[q] = T(0);
+            for (int q = 0; q < 8; ++q) acc[q] = T(0);
             const T* arow = A + (long long)y * pa;
             const bool on = y < n8;
             for (int j0 = 0; j0 < n8; j0 += 8) {
@@ -537,36 +534,30 @@ __global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list
 #pragma unroll
                     for (int u = 0; u < 8; ++u) v[u] = T(0);
                 }
-                block_acc<T, OB>(acc, v, kt + (x0 - j0 + n + kKPad - 7));
+                block8(acc, v, kt + (x0 - j0 + n + kKPad - 7));
             }
-            if (on) {
-                if constexpr (OB == 8) store8(B + (long long)y * pa + x0, acc);
-                else {
-#pragma unroll
-                    for (int q = 0; q < OB; ++q) B[(long long)y * pa + x0 + q] = acc[q];
-                }
-            }
+            if (on) store8(B + (long long)y * pa + x0, acc);
         }
         __syncthreads();
         // column pass
-        const int yb = n8 / OB, cg = (n + 63) >> 6;
+        const int yb = n8 >> 3, cg = (n + 63) >> 6;
         T* d = dst + o;
         for (int task = wave; task < cg * yb; task += nwaves) {
             const int c64 = task / yb, by = task - c64 * yb;
-            const int x = c64 * 64 + lane, y0 = by * OB;
+            const int x = c64 * 64 + lane, y0 = by * 8;
             const bool on = x < n;
-            T acc[OB];
+            T acc[8];
 #pragma unroll
-            for (int r = 0; r < OB; ++r) acc[r] = T(0);
+            for (int r = 0; r < 8; ++r) acc[r] = T(0);
             for (int i0 = 0; i0 < n8; i0 += 8) {
                 T v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) v[u] = on ? B[(long long)(i0 + u) * pa + x] : T(0);
-                block_acc<T, OB>(acc, v, kt + (y0 - i0 + n + kKPad - 7));
+                block8(acc, v, kt + (y0 - i0 + n + kKPad - 7));
             }
             if (on) {
 #pragma unroll
-                for (int r = 0; r < OB; ++r)
+                for (int r = 0; r < 8; ++r)
                     if (y0 + r < n) d[(long long)(y0 + r) * n + x] = acc[r];
             }
         }
