@@ -484,6 +484,20 @@ extern "C" int ggx_create(const ggx_settings* s, const ggx_options* o, ggx_ctx**
         delete ctx;
         return GGX_E_CONFIG;
     }
+    {
+        const ggx_options& q = ctx->opt;
+        const char* bad = nullptr;
+        if (q.precision != GGX_PREC_FP64 && q.precision != GGX_PREC_FP32) bad = "precision";
+        else if (q.batching != GGX_BATCH_FUSED && q.batching != GGX_BATCH_PERGRAIN) bad = "batching";
+        else if (q.conv_engine != GGX_CONV_AUTO && q.conv_engine != GGX_CONV_CIRCULANT) bad = "conv_engine";
+        else if (q.use_graph != 0) bad = "use_graph (hipGraph capture of a step is not built)";
+        else if (q.persistent != 0 && q.persistent != 1) bad = "persistent";
+        else if (q.lds_budget < 0 || q.arena_headroom < 1.0) bad = "lds_budget / arena_headroom";
+        if (bad) {
+            delete ctx;
+            return GGX_E_CONFIG;
+        }
+    }
     ctx->esize = ctx->opt.precision == GGX_PREC_FP32 ? 4 : 8;
     ctx->dev = ctx->opt.device;
     hipError_t e = hipSetDevice(ctx->dev);

@@ -138,7 +138,7 @@ typedef struct ggx_options {
     int32_t conv_engine;      /* GGX_CONV_* */
     int32_t device;           /* HIP device ordinal */
     int32_t lds_budget;       /* bytes of LDS a workgroup may use; 0 = device maximum */
-    int32_t use_graph;        /* capture the step in a hipGraph */
+    int32_t use_graph;        /* must be 0: hipGraph capture of a step is not built (GGX_E_CONFIG) */
     int32_t persistent;       /* 1: fixed grids pull grains from device work lists (no host sync);
                                  0: one workgroup per grain, counts read back after scheduling */
     double  arena_headroom;   /* arena capacity = headroom * initial sum of box areas */

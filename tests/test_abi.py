@@ -2,6 +2,7 @@
 import ctypes
 import os
 import re
+import pytest
 
 from gragles2d_amd import _native
 
@@ -38,3 +39,18 @@ def test_errors_without_gpu_are_status_codes(lib):
     # NumberOfParticles == 0 -> GGX_E_CONFIG, no exception, no GPU call
     assert lib.ggx_settings_derive(ctypes.byref(s), ctypes.byref(d)) == -1
     assert lib.ggx_settings_read_xml(b"/nonexistent/parameters.xml", ctypes.byref(s)) == -7
+
+
+@pytest.mark.parametrize("field,value", [("precision", 2), ("batching", 5), ("conv_engine", 2), ("use_graph", 1),
+                                         ("persistent", 3), ("arena_headroom", 0.5)])
+def test_invalid_options_are_rejected_before_any_gpu_call(lib, field, value):
+    s = _native.Settings()
+    lib.ggx_settings_default(ctypes.byref(s))
+    s.number_of_particles = 10
+    s.convolution_mode = 2
+    s.is_isotropic_network = 1
+    o = _native.Options()
+    lib.ggx_options_default(ctypes.byref(o))
+    setattr(o, field, value)
+    ctx = ctypes.c_void_p()
+    assert lib.ggx_create(ctypes.byref(s), ctypes.byref(o), ctypes.byref(ctx)) == -1
