@@ -513,7 +513,7 @@ __device__ __forceinline__ void redistance(T* Ob, int pitch, const T* C, int cp,
 }
 
 template <class T, bool FULL>
-__global__ void __launch_bounds__(256) k_level(LevelArgs<T> A) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_level(LevelArgs<T> A) {
     HIP_DYNAMIC_SHARED(unsigned char, smem)
     const GridConst& G = A.G;
     const int nb = A.nb;
