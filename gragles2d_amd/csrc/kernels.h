@@ -478,36 +478,14 @@ __device__ __forceinline__ void store8(T* p, const T* a) {
 
 // acc[q] += sum_u v[u] * W[q - u + 7], W = 15 consecutive (wave-uniform) table entries
 template <class T>
-__device__ __forceinline__ void load_w(T* W, ctab_ptr<T> w) {
+__device__ __forceinline__ void block8(T* acc, const T* v, ctab_ptr<T> w) {
+    T W[15];
 #pragma unroll
     for (int t = 0; t < 15; ++t) W[t] = w[t];
-}
-
-template <class T>
-__device__ __forceinline__ void fma8(T* acc, const T* v, const T* W) {
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc[q] += v[u] * W[q - u + 7];
-}
-
-// acc += the contributions of input blocks j0 = 0, 8, .., n8 - 8, two register sets in flight:
-// the loads of block j0 + 8 (field values from LDS, coefficients through the scalar cache) are
-// issued before the FMAs of block j0, so their shared wait falls behind the arithmetic.
-template <class T, class LoadV>
-__device__ __forceinline__ void pipelined_pass(T* acc, int n8, LoadV loadv, ctab_ptr<T> w0) {
-    T va[8], vb[8], Wa[15], Wb[15];
-    loadv(0, va);
-    load_w(Wa, w0);
-    for (int j0 = 0; j0 < n8; j0 += 16) {
-        const bool more = j0 + 8 < n8;
-        if (more) { loadv(j0 + 8, vb); load_w(Wb, w0 - (j0 + 8)); }
-        fma8(acc, va, Wa);
-        if (more) {
-            if (j0 + 16 < n8) { loadv(j0 + 16, va); load_w(Wa, w0 - (j0 + 16)); }
-            fma8(acc, vb, Wb);
-        }
-    }
 }
 
 template <class T>
@@ -549,13 +527,15 @@ __global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list
             for (int q = 0; q < 8; ++q) acc[q] = T(0);
             const T* arow = A + (long long)y * pa;
             const bool on = y < n8;
-            pipelined_pass<T>(acc, n8, [&](int j0, T* v) {
+            for (int j0 = 0; j0 < n8; j0 += 8) {
+                T v[8];
                 if (on) load8(arow + j0, v);
                 else {
 #pragma unroll
                     for (int u = 0; u < 8; ++u) v[u] = T(0);
                 }
-            }, kt + (x0 + n + kKPad - 7));
+                block8(acc, v, kt + (x0 - j0 + n + kKPad - 7));
+            }
             if (on) store8(B + (long long)y * pa + x0, acc);
         }
         __syncthreads();
@@ -569,10 +549,12 @@ __global__ void __launch_bounds__(256) k_conv_dense(const int* __restrict__ list
             T acc[8];
 #pragma unroll
             for (int r = 0; r < 8; ++r) acc[r] = T(0);
-            pipelined_pass<T>(acc, n8, [&](int i0, T* v) {
+            for (int i0 = 0; i0 < n8; i0 += 8) {
+                T v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) v[u] = on ? B[(long long)(i0 + u) * pa + x] : T(0);
-            }, kt + (y0 + n + kKPad - 7));
+                block8(acc, v, kt + (y0 - i0 + n + kKPad - 7));
+            }
             if (on) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
