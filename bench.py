@@ -123,7 +123,8 @@ def measured_traffic(workload, precision, kernel):
         return None
     with open(path) as f:
         t = json.load(f)
-    name = "ggx::k_level<double, true>" if kernel.startswith("k_level") else "ggx::k_conv_dense<double>"
+    t_name = "double" if precision == "fp64" else "float"
+    name = f"ggx::k_level<{t_name}, true>" if kernel.startswith("k_level") else f"ggx::k_conv_dense<{t_name}>"
     e = t.get("per_step", {}).get(name)
     return None if e is None else e["hbm_bytes"]
 
