@@ -68,6 +68,13 @@ def test_tiny_and_edge_networks():
     _run(G.Network.from_polygons(tiny), 5, ppg=40)
 
 
+def test_large_boxes_strip_convolution_and_global_tiles():
+    """Boxes above 128 points: the strip convolution, the level kernel with C and the output in
+    global memory, and the lane-0 walk over C (6 grains, 60 points per grain, margin 40)."""
+    net = G.Network.generate(6, 3)
+    _run(net, 3, ppg=60, gb=40)
+
+
 def test_pergrain_launches_match_batched_bitwise():
     net = G.Network.generate(60, 11)
     s = G.default_settings(number_of_particles=len(net), number_of_points_per_grain=20, domain_border_size=7,
