@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     unsigned char* s_tab = (unsigned char*)(s_cand + ccap);  // square cases (boxes up to kTabMax)
 
     __shared__ int s_bb[4];
-    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len
+    __shared__ int s_int[8];      // 0 start, 1 nmoves, 2 state, 3 err, 4 regular, 5 exists, 6 len, 7 stop
     __shared__ double s_dbl[8];   // 0 volume sum, 1 minx, 2 maxx, 3 miny, 4 maxy
     // the comparison list's boxes, in the output tile's space (free until the contour)
     long long* s_off = (long long*)obase;
@@ -907,6 +907,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
             atomicAdd(A.out_total, (unsigned long long)nbx.n * (unsigned long long)nbx.n);
             s_nbx = nbx;
             s_onext = (long long)fo;
+            s_int[7] = len - 1;  // first failed probe (none yet)
         }
         __syncthreads();
         // ---------------- direct neighbours (only after a regular motion) ----------------
@@ -944,48 +945,59 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                     }
                 }
                 s_cand[i] = ok ? cand : -1;
+                if (!ok) atomicMin(&s_int[7], i);
             }
         }
         __syncthreads();
-        if (threadIdx.x < 64) {
-            // wave 0 walks the contour in order: perimeter (computePerimeter,
-            // grainBoundary.cpp:224-231); after a regular motion also the neighbour list in
-            // first-encounter order with its per-neighbour lengths (lane e holds neighbour e),
-            // the stop at the first failed probe, and the energy (computeEnergy, 215-222)
-            const int lane = threadIdx.x & 63;
+        {
+            // Three ordered sums over the contour, each on its own wave (so on its own SIMD), in
+            // contour order as the reference accumulates them: wave 1 the perimeter
+            // (computePerimeter, grainBoundary.cpp:224-231); after a regular motion wave 2 the
+            // energy (computeEnergy, 215-222; segments from the first failed probe on count
+            // negatively) and wave 0 the neighbour list in first-encounter order with its
+            // per-neighbour lengths (lane e holds neighbour e), up to the first failed probe.
+            const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
             const int L1 = len - 1;
-            double per = 0.0, e = 0.0, my_len = 0.0;
-            int my_id = -1, cnt2 = 0, over = 0, stop = L1;
-            bool stopped = false;
-            for (int c0 = 0; c0 < L1; c0 += 64) {
-                const int i = c0 + lane;
-                const double lv = i < L1 ? s_len[i] : 0.0;
-                const int cv = (nbs && i < L1) ? s_cand[i] : 0;
-                const int m = min(64, L1 - c0);
-                for (int k = 0; k < m; ++k) {
-                    const double l = readlane_f64(lv, k);
-                    per += l * G.h;
-                    if (nbs) {
+            const int stop = __builtin_amdgcn_readfirstlane(nbs ? s_int[7] : L1);
+            const double h = G.h;
+            if (wave == 1) {
+                double per = 0.0;
+                for (int c0 = 0; c0 < L1; c0 += 64) {
+                    const double lv = c0 + lane < L1 ? s_len[c0 + lane] : 0.0;
+                    const int m = min(64, L1 - c0);
+                    for (int k = 0; k < m; ++k) per += readlane_f64(lv, k) * h;
+                }
+                if (lane == 0) A.O.perimeter[g] = per;
+            } else if (wave == 2 && nbs) {
+                double e = 0.0;
+                for (int c0 = 0; c0 < L1; c0 += 64) {
+                    const double lv = c0 + lane < L1 ? s_len[c0 + lane] : 0.0;
+                    const int m = min(64, L1 - c0);
+                    for (int k = 0; k < m; ++k) e += readlane_f64(lv, k) * (c0 + k >= stop ? -1.0 : 1.0) * h;
+                }
+                if (lane == 0) A.O.energy[g] = e;
+            } else if (wave == 0 && nbs) {
+                double my_len = 0.0;
+                int my_id = -1, cnt2 = 0, over = 0;
+                for (int c0 = 0; c0 < stop; c0 += 64) {
+                    const int i = c0 + lane;
+                    const double lv = i < stop ? s_len[i] : 0.0;
+                    const int cv = i < stop ? s_cand[i] : 0;
+                    const int m = min(64, stop - c0);
+                    for (int k = 0; k < m; ++k) {
                         const int cd = __builtin_amdgcn_readlane(cv, k);
-                        if (!stopped && cd < 0) { stopped = true; stop = c0 + k; }
-                        if (!stopped) {
-                            const double ll = l * G.h;
-                            if (__ballot(my_id == cd) == 0ull) {
-                                if (cnt2 < kNbCap) {
-                                    if (lane == cnt2) { my_id = cd; my_len = 0.0; }
-                                    cnt2++;
-                                } else {
-                                    over = 1;
-                                }
+                        const double ll = readlane_f64(lv, k) * h;
+                        if (__ballot(my_id == cd) == 0ull) {
+                            if (cnt2 < kNbCap) {
+                                if (lane == cnt2) { my_id = cd; my_len = 0.0; }
+                                cnt2++;
+                            } else {
+                                over = 1;
                             }
-                            if (my_id == cd) my_len += ll;
                         }
-                        e += l * (stopped ? -1.0 : 1.0) * G.h;
+                        if (my_id == cd) my_len += ll;
                     }
                 }
-            }
-            if (lane == 0) A.O.perimeter[g] = per;
-            if (nbs) {
                 if (lane < cnt2) {
                     A.O.nb_id[(size_t)g * kNbCap + lane] = my_id;
                     A.O.nb_len[(size_t)g * kNbCap + lane] = my_len;
@@ -994,7 +1006,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                     A.O.nb_cnt[g] = cnt2;
                     if (over) A.O.err[g] = kErrNbCap;
                     if (stop < L1) A.O.flags[g] &= ~kRegular;  // markAsInvalidMotion
-                    A.O.energy[g] = e;
                     double newVolume = fabs(fabs(s_dbl[0]) * G.h * G.h * 0.5);
                     double dA = newVolume - old_vol;
                     dA = dA / G.dt;
