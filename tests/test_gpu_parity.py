@@ -1,6 +1,6 @@
 """GPU parity: libggx (HIP, through the C-ABI) against the CPU oracle on the same seeded networks.
 
-Bar (BASELINE north star): FP64 signed-distance fields within 1e-10 max-abs per grain after every
+Bar: FP64 signed-distance fields within 1e-10 max-abs per grain after every
 step, identical surviving-grain sets, identical box bounds, identical neighbour lists (IDs in
 first-encounter order; boundary lengths within 1e-9), identical comparison lists; FP32 grain areas
 within 1e-5 relative.
